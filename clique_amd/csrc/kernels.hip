@@ -1,0 +1,720 @@
+// Device kernels of libcliqg for gfx950 (CDNA4). See kernels.hpp.
+#include "kernels.hpp"
+
+namespace cliqg {
+
+namespace {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <class R> struct Acc4;
+template <> struct Acc4<double> { using type = d4v; };
+template <> struct Acc4<float> { using type = f4v; };
+
+// 16x16x4 MFMA: lane l supplies A(l%16, l/16) and B(l/16, l%16). Result
+// element r of lane l is C(4*(l/16) + r, l%16) for FP32 but C(l/16 + 4*r, l%16)
+// for FP64 (measured on gfx950 with an in-bounds single-wave probe).
+__device__ __forceinline__ d4v mfma16(double a, double b, d4v c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma16(float a, float b, f4v c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <class T> __device__ __forceinline__ T make_t(typename Scalar<T>::real r,
+                                                       typename Scalar<T>::real i);
+template <> __device__ __forceinline__ float make_t<float>(float r, float) { return r; }
+template <> __device__ __forceinline__ double make_t<double>(double r, double) { return r; }
+template <> __device__ __forceinline__ cfloat make_t<cfloat>(float r, float i) { return {r, i}; }
+template <> __device__ __forceinline__ cdouble make_t<cdouble>(double r, double i) { return {r, i}; }
+
+__device__ __forceinline__ double real_part_only(double a) { return a; }
+__device__ __forceinline__ cdouble real_part_only(cdouble a) { return {a.x, 0.0}; }
+
+__device__ __forceinline__ int find_segment(const int* __restrict__ start, int n, int g) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void record_status(int* status, int code, int front, int col) {
+    if (atomicCAS(status, 0, code) == 0) {
+        status[1] = front;
+        status[2] = col;
+    }
+}
+
+constexpr int TK = 16;
+constexpr int LPAD = 16;
+
+// number of column-major lower-trapezoid tiles before tile column tj
+__host__ __device__ __forceinline__ long long lower_prefix(long long tj, long long tilesM) {
+    return tj * tilesM - tj * (tj - 1) / 2;
+}
+
+}  // namespace
+
+int gemm_tiles(int M, int N, int flags) {
+    if (M <= 0 || N <= 0) return 0;
+    const int tm = (M + GEMM_TM - 1) / GEMM_TM, tn = (N + GEMM_TN - 1) / GEMM_TN;
+    if (flags & G_LOWER) {
+        const int c = tn < tm ? tn : tm;
+        return (int)lower_prefix(c, tm);
+    }
+    return tm * tn;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped GEMM: one 64x64 output tile per workgroup of 4 waves; each wave
+// owns a 32x32 quadrant as 2x2 MFMA 16x16 blocks. Complex products use four
+// real MFMAs on planar (re, im) LDS tiles.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ descs,
+                                              const int* __restrict__ tile_start, int ndesc) {
+    using R = typename Scalar<T>::real;
+    using W = typename Scalar<T>::wide;
+    using A4 = typename Acc4<R>::type;
+    constexpr bool CPX = Scalar<T>::complex;
+    constexpr int NP = CPX ? 2 : 1;
+
+    __shared__ R As[NP][TK][GEMM_TM + LPAD];
+    __shared__ R Bs[NP][TK][GEMM_TN + LPAD];
+
+    const int tile = blockIdx.x;
+    const int di = find_segment(tile_start, ndesc, tile);
+    const GemmDesc<T> d = descs[di];
+    const int t = tile - tile_start[di];
+    const int tilesM = (d.M + GEMM_TM - 1) / GEMM_TM;
+    int ti, tj;
+    if (d.flags & G_LOWER) {
+        const double b2 = 2.0 * tilesM + 1.0;
+        int c = (int)floor((b2 - sqrt(fmax(b2 * b2 - 8.0 * t, 0.0))) * 0.5);
+        if (c < 0) c = 0;
+        while (c > 0 && lower_prefix(c, tilesM) > t) --c;
+        while (lower_prefix(c + 1, tilesM) <= t) ++c;
+        tj = c;
+        ti = c + (int)(t - lower_prefix(c, tilesM));
+    } else {
+        ti = t % tilesM;
+        tj = t / tilesM;
+    }
+    const int row0 = ti * GEMM_TM, col0 = tj * GEMM_TN;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave & 1, wj = wave >> 1;
+    const bool transA = d.flags & G_TRANSA, transB = d.flags & G_TRANSB;
+    const bool conjA = d.flags & G_CONJA, conjB = d.flags & G_CONJB;
+    const W* scaleK = reinterpret_cast<const W*>(d.scaleK);
+
+    A4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            accR[a][b] = A4{0, 0, 0, 0};
+            accI[a][b] = A4{0, 0, 0, 0};
+        }
+
+    T ra[4], rb[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = tid + 256 * r;
+            int i, k;
+            if (!transA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
+            const int gi = row0 + i, gk = k0 + k;
+            T v = zero<T>();
+            if (gi < d.M && gk < d.K) {
+                v = transA ? d.A[gk + (int64_t)gi * d.lda] : d.A[gi + (int64_t)gk * d.lda];
+                if (scaleK) v = v * cvt<T>(scaleK[gk]);
+                if (conjA) v = conj_(v);
+            }
+            ra[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = tid + 256 * r;
+            int j, k;
+            if (transB) { j = e & 63; k = e >> 6; } else { k = e & 15; j = e >> 4; }
+            const int gj = col0 + j, gk = k0 + k;
+            T v = zero<T>();
+            if (gj < d.N && gk < d.K) {
+                v = transB ? d.B[gj + (int64_t)gk * d.ldb] : d.B[gk + (int64_t)gj * d.ldb];
+                if (conjB) v = conj_(v);
+            }
+            rb[r] = v;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = tid + 256 * r;
+            int i, k;
+            if (!transA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
+            As[0][k][i] = re(ra[r]);
+            if constexpr (CPX) As[NP - 1][k][i] = im(ra[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = tid + 256 * r;
+            int j, k;
+            if (transB) { j = e & 63; k = e >> 6; } else { k = e & 15; j = e >> 4; }
+            Bs[0][k][j] = re(rb[r]);
+            if constexpr (CPX) Bs[NP - 1][k][j] = im(rb[r]);
+        }
+    };
+
+    const int nk = (d.K + TK - 1) / TK;
+    if (nk > 0) {
+        load(0);
+        store();
+        __syncthreads();
+    }
+    for (int c = 0; c < nk; ++c) {
+        if (c + 1 < nk) load((c + 1) * TK);
+#pragma unroll
+        for (int kk = 0; kk < TK / 4; ++kk) {
+            const int kr = kk * 4 + (lane >> 4);
+            R aR[2], aI[2], bR[2], bI[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                aR[q] = As[0][kr][wi * 32 + q * 16 + (lane & 15)];
+                bR[q] = Bs[0][kr][wj * 32 + q * 16 + (lane & 15)];
+                if constexpr (CPX) {
+                    aI[q] = As[NP - 1][kr][wi * 32 + q * 16 + (lane & 15)];
+                    bI[q] = Bs[NP - 1][kr][wj * 32 + q * 16 + (lane & 15)];
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    accR[a][b] = mfma16(aR[a], bR[b], accR[a][b]);
+                    if constexpr (CPX) {
+                        accR[a][b] = mfma16(-aI[a], bI[b], accR[a][b]);
+                        accI[a][b] = mfma16(aR[a], bI[b], accI[a][b]);
+                        accI[a][b] = mfma16(aI[a], bR[b], accI[a][b]);
+                    }
+                }
+        }
+        __syncthreads();
+        if (c + 1 < nk) {
+            store();
+            __syncthreads();
+        }
+    }
+
+    const W* scaleN = reinterpret_cast<const W*>(d.scaleN);
+    const bool lower = d.flags & G_LOWER, accum = d.flags & G_ACCUM, neg = d.flags & G_NEG;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                constexpr bool F64 = sizeof(R) == 8;
+                const int rin = F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+                const int gi = row0 + wi * 32 + a * 16 + rin;
+                const int gj = col0 + wj * 32 + b * 16 + (lane & 15);
+                if (gi >= d.M || gj >= d.N || (lower && gi < gj)) continue;
+                T v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
+                if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
+                if (neg) v = -v;
+                T* p = d.C + gi + (int64_t)gj * d.ldc;
+                if (accum) v = *p + v;
+                *p = v;
+            }
+}
+
+template <class T>
+void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
+                 hipStream_t s) {
+    if (ntiles <= 0 || ndesc <= 0) return;
+    hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start, ndesc);
+}
+
+// ---------------------------------------------------------------------------
+// Clearing of update matrices (one 1024-element chunk per workgroup)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_segments(const ZeroSeg<T>* __restrict__ segs,
+                                                       const int* __restrict__ chunk_start,
+                                                       int nseg) {
+    const int g = blockIdx.x;
+    const int si = find_segment(chunk_start, nseg, g);
+    const ZeroSeg<T> sg = segs[si];
+    const int64_t base = (int64_t)(g - chunk_start[si]) * 1024;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+        const int64_t e = base + i;
+        if (e < sg.n) sg.p[e] = zero<T>();
+    }
+}
+
+template <class T>
+void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, int nseg,
+                          int nchunks, hipStream_t s) {
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL(k_zero_segments<T>, dim3(nchunks), dim3(256), 0, s, d_segs, d_chunk_start,
+                       nseg);
+}
+
+// ---------------------------------------------------------------------------
+// Assembly of A into the fronts (scatter through a precomputed map)
+// ---------------------------------------------------------------------------
+template <class T, class V>
+__global__ void k_assemble(const V* __restrict__ vals, const int64_t* __restrict__ dst,
+                           int64_t nnz, T* __restrict__ pool, bool conj) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = dst[e];
+        if (t >= 0) pool[t] = cvt<T>(cj(vals[e], conj));
+    }
+}
+
+template <class T, class V>
+void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bool conj,
+                     hipStream_t s) {
+    if (nnz <= 0) return;
+    int64_t blocks = (nnz + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL((k_assemble<T, V>), dim3((unsigned)blocks), dim3(256), 0, s, vals, dst,
+                       nnz, pool, conj);
+}
+
+// ---------------------------------------------------------------------------
+// Extend-add: one workgroup per (child, update column). Children of one
+// slot have distinct parents, so no two workgroups touch the same entry.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restrict__ descs,
+                                                    const int* __restrict__ col_start, int nd) {
+    const int g = blockIdx.x;
+    const int c = find_segment(col_start, nd, g);
+    const ExtendDesc<T> d = descs[c];
+    const int j = g - col_start[c];
+    const int J = d.rel[j];
+    const T* col = d.Uc + (int64_t)j * d.uc;
+    T* dst;
+    if (J < d.np) dst = d.Lp + (int64_t)J * d.mp;
+    else dst = d.Up + (int64_t)(J - d.np) * d.up - d.np;
+    for (int i = j + threadIdx.x; i < d.uc; i += blockDim.x) {
+        T* p = dst + d.rel[i];
+        *p = *p + col[i];
+    }
+}
+
+template <class T>
+void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int ndesc,
+                       int ncols, hipStream_t s) {
+    if (ncols <= 0) return;
+    hipLaunchKernelGGL(k_extend_add<T>, dim3(ncols), dim3(256), 0, s, d_descs, d_col_start,
+                       ndesc);
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block: unpivoted LDL^T of a b x b block (b <= NB) in FP64 (field
+// type) in LDS, unit lower inverse of its L. One workgroup per front.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
+                                                  bool herm, int* status) {
+    using W = typename Scalar<T>::wide;
+    extern __shared__ unsigned char smem_raw[];
+    W (*a)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw);
+    W (*li)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw + sizeof(W) * NB * (NB + 1));
+
+    const DiagDesc<T> d = descs[blockIdx.x];
+    const int b = d.b, tid = threadIdx.x;
+    for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
+        const int i = idx % NB, j = idx / NB;
+        W v = W{};
+        if (i < b && j < b && i >= j) v = cvt<W>(d.A11[i + (int64_t)j * d.lda]);
+        a[i][j] = v;
+        li[i][j] = W{};
+    }
+    W* Dout = reinterpret_cast<W*>(d.D);
+    for (int j = 0; j < b; ++j) {
+        __syncthreads();
+        W piv = a[j][j];
+        if (herm) piv = real_part_only(piv);
+        if (tid == 0) {
+            if (is_zero_or_nonfinite(piv)) record_status(status, 2, d.front, d.col0 + j);
+            Dout[j] = piv;
+        }
+        for (int i = j + 1 + tid; i < b; i += blockDim.x) a[i][j] = a[i][j] / piv;
+        __syncthreads();
+        const int r = b - j - 1;
+        for (int idx = tid; idx < r * r; idx += blockDim.x) {
+            const int ii = j + 1 + idx % r, cc = j + 1 + idx / r;
+            if (cc <= ii) a[ii][cc] = a[ii][cc] - a[ii][j] * piv * cj(a[cc][j], herm);
+        }
+    }
+    __syncthreads();
+    // unit lower inverse, rows in sequence, columns in parallel
+    if (tid < b) li[tid][tid] = one<W>();
+    for (int i = 1; i < b; ++i) {
+        __syncthreads();
+        if (tid < i) {
+            const int j = tid;
+            W s = W{};
+            for (int l = j; l < i; ++l) s = s + a[i][l] * li[l][j];
+            li[i][j] = -s;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < b * b; idx += blockDim.x) {
+        const int i = idx % b, j = idx / b;
+        if (i > j) d.A11[i + (int64_t)j * d.lda] = cvt<T>(a[i][j]);
+        else if (i == j) d.A11[i + (int64_t)j * d.lda] = one<T>();
+    }
+    for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
+        const int i = idx % NB, j = idx / NB;
+        d.Linv[i + j * NB] = cvt<T>(li[i][j]);
+    }
+}
+
+template <class T>
+void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
+                     hipStream_t s) {
+    if (ndesc <= 0) return;
+    using W = typename Scalar<T>::wide;
+    const size_t shm = 2 * sizeof(W) * NB * (NB + 1);
+    static bool attr_set = false;
+    if (!attr_set) {
+        CQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag_ldl<T>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_diag_ldl<T>, dim3(ndesc), dim3(256), shm, s, d_descs, herm, d_status);
+}
+
+// ---------------------------------------------------------------------------
+// Solve helpers
+// ---------------------------------------------------------------------------
+template <class T, class V>
+__global__ void k_pull(const V* __restrict__ x, int64_t ldx, const int* __restrict__ inv,
+                       T* __restrict__ xn, int N, int k) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(g % N), j = (int)(g / N);
+        xn[g] = cvt<T>(x[inv[p] + j * ldx]);
+    }
+}
+template <class T, class V>
+__global__ void k_push(const T* __restrict__ xn, const int* __restrict__ inv, V* __restrict__ x,
+                       int64_t ldx, int N, int k) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(g % N), j = (int)(g / N);
+        x[inv[p] + j * ldx] = cvt<V>(xn[g]);
+    }
+}
+
+static inline unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+template <class T, class V>
+void launch_pull(const V* x, int64_t ldx, const int* inv_map, T* xn, int N, int k, hipStream_t s) {
+    hipLaunchKernelGGL((k_pull<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, x, ldx,
+                       inv_map, xn, N, k);
+}
+template <class T, class V>
+void launch_push(const T* xn, const int* inv_map, V* x, int64_t ldx, int N, int k, hipStream_t s) {
+    hipLaunchKernelGGL((k_push<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, xn,
+                       inv_map, x, ldx, N, k);
+}
+
+template <class T>
+__global__ void k_copy_in(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
+                          int nseg, int nrows, T* __restrict__ wpool, const T* __restrict__ xn,
+                          int N, int k, const int* __restrict__ lowpool, bool backward) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrows) return;
+    const int si = find_segment(row_start, nseg, g);
+    const SolveSeg sg = segs[si];
+    const int i = g - row_start[si];
+    T* w = wpool + sg.w * k;
+    if (i < sg.n) {
+        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = xn[sg.off + i + (int64_t)j * N];
+    } else if (backward) {
+        const int src = lowpool[sg.low + (i - sg.n)];
+        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = xn[src + (int64_t)j * N];
+    } else {
+        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = zero<T>();
+    }
+}
+template <class T>
+void launch_copy_in(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
+                    const T* xn, int N, int k, const int* lowpool, bool backward, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_copy_in<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs, row_start,
+                       nseg, nrows, wpool, xn, N, k, lowpool, backward);
+}
+
+template <class T>
+__global__ void k_copy_out(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
+                           int nseg, int nrows, const T* __restrict__ wpool, T* __restrict__ xn,
+                           int N, int k) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrows) return;
+    const int si = find_segment(row_start, nseg, g);
+    const SolveSeg sg = segs[si];
+    const int i = g - row_start[si];
+    if (i >= sg.n) return;
+    const T* w = wpool + sg.w * k;
+    for (int j = 0; j < k; ++j) xn[sg.off + i + (int64_t)j * N] = w[i + (int64_t)j * sg.m];
+}
+template <class T>
+void launch_copy_out(const SolveSeg* segs, const int* row_start, int nseg, int nrows,
+                     const T* wpool, T* xn, int N, int k, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_copy_out<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs,
+                       row_start, nseg, nrows, wpool, xn, N, k);
+}
+
+template <class T>
+__global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __restrict__ row_start,
+                               int nd, int nrows, T* __restrict__ wpool,
+                               const int* __restrict__ relpool, int k) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrows) return;
+    const int c = find_segment(row_start, nd, g);
+    const SolveExtend d = ds[c];
+    const int i = g - row_start[c];
+    const T* wc = wpool + d.wc * k;
+    T* wp = wpool + d.wp * k;
+    const int I = relpool[d.rel + i];
+    for (int j = 0; j < k; ++j) {
+        T* p = wp + I + (int64_t)j * d.mp;
+        *p = *p + wc[d.nc + i + (int64_t)j * d.mc];
+    }
+}
+template <class T>
+void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
+                         T* wpool, const int* relpool, int k, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_solve_extend<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, d,
+                       row_start, nd, nrows, wpool, relpool, k);
+}
+
+template <class T, class V>
+__global__ void k_diag_solve(T* __restrict__ xn, const V* __restrict__ D, int N, int k,
+                             int* status) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int p = (int)(g % N);
+        const V dv = D[p];
+        if (is_zero_or_nonfinite(dv)) record_status(status, 2, -1, p);
+        xn[g] = cvt<T>(cvt<V>(xn[g]) / dv);
+    }
+}
+template <class T, class V>
+void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream_t s) {
+    hipLaunchKernelGGL((k_diag_solve<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, xn,
+                       D, N, k, d_status);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 sparse kernels
+// ---------------------------------------------------------------------------
+template <class V>
+__global__ void k_spmv(int N, const int64_t* __restrict__ rowoff, const int* __restrict__ cols,
+                       const V* __restrict__ vals, double alpha, const V* __restrict__ x,
+                       int64_t ldx, double beta, V* __restrict__ y, int64_t ldy, int k) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(g % N), j = (int)(g / N);
+        V acc = V{};
+        for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) acc = acc + vals[e] * x[cols[e] + j * ldx];
+        V* py = y + i + j * ldy;
+        V out = scale(acc, alpha);
+        if (beta != 0.0) out = out + scale(*py, beta);
+        *py = out;
+    }
+}
+template <class V>
+void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
+                 const V* x, int64_t ldx, double beta, V* y, int64_t ldy, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, N, rowoff, cols,
+                       vals, alpha, x, ldx, beta, y, ldy, k);
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+template <class V>
+__global__ void k_row_abs_sum_max(int N, const int64_t* __restrict__ rowoff,
+                                  const V* __restrict__ vals, unsigned long long* out) {
+    __shared__ double red[256];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) s += abs1(vals[e]);
+        m = fmax(m, s);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomic_max_nonneg(out, red[0]);
+}
+template <class V>
+void launch_row_abs_sum_max(int N, const int64_t* rowoff, const V* vals, unsigned long long* out,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_row_abs_sum_max<V>, dim3(grid_for(N) > 1024 ? 1024 : grid_for(N)),
+                       dim3(256), 0, s, N, rowoff, vals, out);
+}
+
+template <class V>
+__global__ void k_col_abs_max(const V* __restrict__ x, int64_t ldx, int N,
+                              unsigned long long* out) {
+    __shared__ double red[256];
+    const int j = blockIdx.y;
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        m = fmax(m, abs1(x[i + j * ldx]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomic_max_nonneg(out + j, red[0]);
+}
+template <class V>
+void launch_col_abs_max(const V* x, int64_t ldx, int N, int k, unsigned long long* out,
+                        hipStream_t s) {
+    unsigned gx = grid_for(N);
+    if (gx > 256) gx = 256;
+    hipLaunchKernelGGL(k_col_abs_max<V>, dim3(gx, k), dim3(256), 0, s, x, ldx, N, out);
+}
+
+template <class V>
+__global__ void k_axpy_cols(V* __restrict__ y, int64_t ldy, const V* __restrict__ x, int64_t ldx,
+                            int N, int k) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(g % N), j = (int)(g / N);
+        y[i + j * ldy] = y[i + j * ldy] + x[i + j * ldx];
+    }
+}
+template <class V>
+void launch_axpy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_axpy_cols<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, y, ldy, x,
+                       ldx, N, k);
+}
+
+template <class V>
+__global__ void k_copy_cols(V* __restrict__ y, int64_t ldy, const V* __restrict__ x, int64_t ldx,
+                            int N, int k) {
+    const int64_t total = (int64_t)N * k;
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(g % N), j = (int)(g / N);
+        y[i + j * ldy] = x[i + j * ldx];
+    }
+}
+template <class V>
+void launch_copy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_cols<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, y, ldy, x,
+                       ldx, N, k);
+}
+
+template <class V>
+__global__ void k_adjoint_dA(int N, const int64_t* __restrict__ rowoff, const int* __restrict__ cols,
+                             const V* __restrict__ lam, int64_t ldl, const V* __restrict__ x,
+                             int64_t ldx, int k, bool conj, V* __restrict__ dA) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) {
+            const int c = cols[e];
+            V acc = V{};
+            for (int r = 0; r < k; ++r) acc = acc + lam[i + r * ldl] * cj(x[c + r * ldx], conj);
+            dA[e] = -acc;
+        }
+    }
+}
+template <class V>
+void launch_adjoint_dA(int N, const int64_t* rowoff, const int* cols, const V* lambda,
+                       int64_t ldl, const V* x, int64_t ldx, int k, bool conj, V* dA,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_adjoint_dA<V>, dim3(grid_for(N)), dim3(256), 0, s, N, rowoff, cols, lambda,
+                       ldl, x, ldx, k, conj, dA);
+}
+
+template <class T, class V>
+__global__ void k_widen(const T* __restrict__ src, V* __restrict__ dst, int64_t n) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n;
+         g += (int64_t)gridDim.x * blockDim.x)
+        dst[g] = cvt<V>(src[g]);
+}
+template <class T, class V>
+void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_widen<T, V>), dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+// ---------------------------------------------------------------------------
+#define CQ_INST_T(T)                                                                           \
+    template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
+    template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, hipStream_t);        \
+    template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
+    template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \
+    template void launch_copy_in<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,   \
+                                    int, const int*, bool, hipStream_t);                        \
+    template void launch_copy_out<T>(const SolveSeg*, const int*, int, int, const T*, T*, int,  \
+                                     int, hipStream_t);                                         \
+    template void launch_solve_extend<T>(const SolveExtend*, const int*, int, int, T*,          \
+                                         const int*, int, hipStream_t);
+
+CQ_INST_T(float)
+CQ_INST_T(double)
+CQ_INST_T(cfloat)
+CQ_INST_T(cdouble)
+
+#define CQ_INST_TV(T, V)                                                                        \
+    template void launch_assemble<T, V>(const V*, const int64_t*, int64_t, T*, bool, hipStream_t); \
+    template void launch_pull<T, V>(const V*, int64_t, const int*, T*, int, int, hipStream_t);   \
+    template void launch_push<T, V>(const T*, const int*, V*, int64_t, int, int, hipStream_t);   \
+    template void launch_diag_solve<T, V>(T*, const V*, int, int, int*, hipStream_t);           \
+    template void launch_widen<T, V>(const T*, V*, int64_t, hipStream_t);
+
+CQ_INST_TV(float, double)
+CQ_INST_TV(double, double)
+CQ_INST_TV(cfloat, cdouble)
+CQ_INST_TV(cdouble, cdouble)
+
+#define CQ_INST_V(V)                                                                            \
+    template void launch_spmv<V>(int, const int64_t*, const int*, const V*, double, const V*,   \
+                                 int64_t, double, V*, int64_t, int, hipStream_t);              \
+    template void launch_row_abs_sum_max<V>(int, const int64_t*, const V*, unsigned long long*, \
+                                            hipStream_t);                                      \
+    template void launch_col_abs_max<V>(const V*, int64_t, int, int, unsigned long long*,       \
+                                        hipStream_t);                                          \
+    template void launch_axpy_cols<V>(V*, int64_t, const V*, int64_t, int, int, hipStream_t);   \
+    template void launch_copy_cols<V>(V*, int64_t, const V*, int64_t, int, int, hipStream_t);   \
+    template void launch_adjoint_dA<V>(int, const int64_t*, const int*, const V*, int64_t,      \
+                                       const V*, int64_t, int, bool, V*, hipStream_t);
+
+CQ_INST_V(double)
+CQ_INST_V(cdouble)
+
+}  // namespace cliqg

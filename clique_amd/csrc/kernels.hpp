@@ -1,0 +1,147 @@
+// Device kernels of libcliqg (gfx950) and their launchers.
+//
+// All work of one tree level is issued as a handful of grouped launches
+// whose work lists (descriptor arrays) are built once per plan, so the
+// factorization and the solves are fixed launch sequences.
+#pragma once
+#include "common.hpp"
+
+namespace cliqg {
+
+// Width of a diagonal block (panel) of the partial LDL^T.
+constexpr int NB = 64;
+// Tile of the grouped GEMM.
+constexpr int GEMM_TM = 64;
+constexpr int GEMM_TN = 64;
+
+enum GemmFlags : int {
+    G_TRANSA = 1,   // op(A)(i,k) = A[k + i*lda]   (else A[i + k*lda])
+    G_TRANSB = 2,   // op(B)(k,j) = B[j + k*ldb]   (else B[k + j*ldb])
+    G_CONJA = 4,
+    G_CONJB = 8,
+    G_LOWER = 16,   // compute/write only C(i,j) with i >= j
+    G_ACCUM = 32,   // C := C + alpha*op(A)op(B)  (else C := alpha*op(A)op(B))
+    G_NEG = 64,     // alpha = -1 (else +1)
+};
+
+// C(MxN) (+)= +-op(A)(MxK) diag(scaleK) op(B)(KxN) [diag(scaleN)^{-1}]
+template <class T>
+struct GemmDesc {
+    const T* A;
+    const T* B;
+    T* C;
+    const void* scaleK;   // FP64 field-type pivots multiplying op(A)'s column k (or null)
+    const void* scaleN;   // FP64 field-type pivots dividing output column j (or null)
+    int64_t lda, ldb, ldc;
+    int M, N, K;
+    int flags;
+};
+
+// Child-to-parent extend-add of one child's update matrix
+// (reference ldl/local.hpp:60-102).
+template <class T>
+struct ExtendDesc {
+    const T* Uc;      // uc x uc, ld uc, lower triangle
+    const int* rel;   // uc relative indices into the parent's front
+    T* Lp;            // parent frontL (mp x np, ld mp)
+    T* Up;            // parent update matrix (up x up, ld up)
+    int uc, np, mp, up;
+};
+
+// One front of a diagonal-block factorization step.
+template <class T>
+struct DiagDesc {
+    T* A11;          // frontL + k + k*m
+    void* D;         // FP64 pivots of this block (field type)
+    T* Linv;         // NB x NB inverse of the unit lower block (ld NB)
+    int64_t lda;
+    int b;
+    int front;       // for error reports
+    int col0;        // k, for error reports
+};
+
+// Per-front segment of a solve-workspace operation.
+struct SolveSeg {
+    int64_t w;       // offset of W_s (elements, for k = 1; times k, ld = m)
+    int64_t low;     // offset of lowerStruct in the index pool
+    int n, m, off, pad;
+};
+
+// Child-to-parent extend-add of solve workspaces (lower_solve/local.hpp:83-104)
+struct SolveExtend {
+    int64_t wc;      // child workspace offset
+    int64_t wp;      // parent workspace offset
+    int64_t rel;     // offset of the child's relative indices
+    int nc, mc, uc, mp;
+};
+
+// A contiguous range to clear.
+template <class T>
+struct ZeroSeg {
+    T* p;
+    int64_t n;
+};
+
+// ---- launchers (stream-ordered, asynchronous) ----------------------------------
+// Clears every segment; chunk_start[i] = number of 1024-element chunks before
+// segment i.
+template <class T>
+void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, int nseg,
+                          int nchunks, hipStream_t s);
+template <class T>
+void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
+                 hipStream_t s);
+// number of output tiles of a descriptor (host side, must match the kernel)
+int gemm_tiles(int M, int N, int flags);
+
+template <class T, class V>
+void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bool conj,
+                     hipStream_t s);
+
+template <class T>
+void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int ndesc,
+                       int ncols, hipStream_t s);
+
+template <class T>
+void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
+                     hipStream_t s);
+
+// solve helpers; T working type, V FP64 field type
+template <class T, class V>
+void launch_pull(const V* x, int64_t ldx, const int* inv_map, T* xn, int N, int k, hipStream_t s);
+template <class T, class V>
+void launch_push(const T* xn, const int* inv_map, V* x, int64_t ldx, int N, int k, hipStream_t s);
+template <class T>
+void launch_copy_in(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
+                    const T* xn, int N, int k, const int* lowpool, bool backward, hipStream_t s);
+template <class T>
+void launch_copy_out(const SolveSeg* segs, const int* row_start, int nseg, int nrows,
+                     const T* wpool, T* xn, int N, int k, hipStream_t s);
+template <class T>
+void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
+                         T* wpool, const int* relpool, int k, hipStream_t s);
+template <class T, class V>
+void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream_t s);
+
+// FP64 sparse kernels (original ordering, CSR)
+template <class V>
+void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
+                 const V* x, int64_t ldx, double beta, V* y, int64_t ldy, int k, hipStream_t s);
+template <class V>
+void launch_row_abs_sum_max(int N, const int64_t* rowoff, const V* vals, unsigned long long* out,
+                            hipStream_t s);
+template <class V>
+void launch_col_abs_max(const V* x, int64_t ldx, int N, int k, unsigned long long* out,
+                        hipStream_t s);
+template <class V>
+void launch_axpy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s);
+template <class V>
+void launch_copy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s);
+template <class V>
+void launch_adjoint_dA(int N, const int64_t* rowoff, const int* cols, const V* lambda,
+                       int64_t ldl, const V* x, int64_t ldx, int k, bool conj, V* dA,
+                       hipStream_t s);
+template <class T, class V>
+void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s);
+
+}  // namespace cliqg

@@ -1,0 +1,764 @@
+// Device plan of the multifrontal LDL^T engine (host orchestration).
+//
+// Memory layout in HBM (one allocation per pool):
+//   factor pool  : front s is an m x n column-major block (ld m) at L_s,
+//                  the reference's frontL (dist_symm_front_tree/decl.hpp:174-185)
+//   update pool  : front s's u x u Schur complement (ld u) at U_s, the
+//                  reference's `work` matrix during LDL (ldl/local.hpp:42-51)
+//   pivot pool   : D in nodal order, FP64 field type for every precision
+//   inverse pool : per 64-wide diagonal block of every front, the inverse of
+//                  its unit lower triangle (64 x 64, ld 64)
+//   workspace    : per front an m x k solve workspace W_s (ld m)
+// The schedule is level-synchronous: fronts of equal height are processed by
+// the same grouped launches; its descriptor arrays are built once.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace cliqg {
+
+void DevBuf::alloc(size_t n) {
+    release();
+    if (n) CQ_HIP(hipMalloc(&p, n));
+    bytes = n;
+}
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+namespace {
+
+template <class X>
+void upload(DevBuf& b, const std::vector<X>& v) {
+    b.alloc(std::max<size_t>(v.size(), 1) * sizeof(X));
+    if (!v.empty()) CQ_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+}
+
+// Descriptor arrays of a schedule, staged on the host and uploaded at once.
+struct Arena {
+    std::vector<char> host;
+    DevBuf dev;
+    template <class X>
+    size_t add(const std::vector<X>& v) {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + std::max<size_t>(v.size(), 1) * sizeof(X));
+        if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(X));
+        return off;
+    }
+    void finish() {
+        dev.alloc(std::max<size_t>(host.size(), 1));
+        if (!host.empty())
+            CQ_HIP(hipMemcpy(dev.p, host.data(), host.size(), hipMemcpyHostToDevice));
+        std::vector<char>().swap(host);
+    }
+    template <class X> const X* at(size_t off) const {
+        return reinterpret_cast<const X*>(static_cast<const char*>(dev.p) + off);
+    }
+};
+
+// First-fit allocator over a virtual range with coalescing free list.
+struct IntervalAlloc {
+    std::map<int64_t, int64_t> freeList;   // offset -> size
+    int64_t top = 0;
+    int64_t alloc(int64_t n) {
+        if (n <= 0) return 0;
+        for (auto it = freeList.begin(); it != freeList.end(); ++it) {
+            if (it->second >= n) {
+                const int64_t off = it->first, rest = it->second - n;
+                freeList.erase(it);
+                if (rest > 0) freeList[off + n] = rest;
+                return off;
+            }
+        }
+        // grow, absorbing a free block that ends at the top
+        if (!freeList.empty()) {
+            auto last = std::prev(freeList.end());
+            if (last->first + last->second == top) {
+                const int64_t off = last->first;
+                freeList.erase(last);
+                top = off + n;
+                return off;
+            }
+        }
+        const int64_t off = top;
+        top += n;
+        return off;
+    }
+    void release(int64_t off, int64_t n) {
+        if (n <= 0) return;
+        auto it = freeList.emplace(off, n).first;
+        if (it != freeList.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                freeList.erase(it);
+                it = prev;
+            }
+        }
+        auto next = std::next(it);
+        if (next != freeList.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            freeList.erase(next);
+        }
+    }
+};
+
+enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_COPYIN, S_COPYOUT, S_SEXTEND, S_ZERO };
+
+struct Step {
+    int kind;
+    size_t desc, start;   // arena offsets
+    int ndesc, nwork;
+    bool backward;
+};
+
+template <class T>
+struct StepBuilder {
+    Arena& arena;
+    std::vector<Step>& steps;
+
+    void gemm(const std::vector<GemmDesc<T>>& in) {
+        std::vector<GemmDesc<T>> ds;
+        std::vector<int> start;
+        int total = 0;
+        for (const auto& d : in) {
+            const int t = gemm_tiles(d.M, d.N, d.flags);
+            if (t == 0) continue;
+            if (d.K == 0 && (d.flags & G_ACCUM)) continue;
+            ds.push_back(d);
+            start.push_back(total);
+            total += t;
+        }
+        if (ds.empty()) return;
+        Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false};
+        steps.push_back(s);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Numeric state for one working type T (float, double, cfloat, cdouble)
+// ---------------------------------------------------------------------------
+template <class T>
+struct Numeric : NumericBase {
+    using V = typename Scalar<T>::wide;
+    Plan& P;
+    DevBuf fpool, upool, dpool, linv;
+    Arena farena;
+    std::vector<Step> fsteps;
+
+    struct SolveSched {
+        Arena arena;
+        std::vector<Step> fwd, bwd;
+        DevBuf wpool, xn;
+    };
+    std::map<int, std::unique_ptr<SolveSched>> solves;
+    DevBuf rbuf, identity;
+    int rbufCols = 0;
+
+    explicit Numeric(Plan& p) : P(p) {
+        fpool.alloc(std::max<int64_t>(P.factorEntries, 1) * sizeof(T));
+        upool.alloc(std::max<int64_t>(P.updateEntries, 1) * sizeof(T));
+        dpool.alloc(std::max(P.N, 1) * sizeof(V));
+        linv.alloc(std::max<int64_t>(P.linvEntries, 1) * sizeof(T));
+        build_factor_schedule();
+    }
+
+    size_t device_bytes() const override {
+        size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
+                   rbuf.bytes + identity.bytes;
+        for (auto& kv : solves)
+            b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes;
+        return b;
+    }
+
+    T* L(int s) const { return fpool.as<T>() + P.fronts[s].L; }
+    T* U(int s) const { return upool.as<T>() + P.fronts[s].U; }
+    V* D(int s) const { return dpool.as<V>() + P.fronts[s].off; }
+    T* Li(int s, int k0) const { return linv.as<T>() + P.fronts[s].Linv + (int64_t)(k0 / NB) * NB * NB; }
+
+    void build_factor_schedule() {
+        StepBuilder<T> sb{farena, fsteps};
+        const int herm = P.conjugate ? G_CONJB : 0;
+        for (int h = 0; h < P.numLevels; ++h) {
+            const auto& lv = P.levels[h];
+            {
+                std::vector<ZeroSeg<T>> zs;
+                std::vector<int> start;
+                int chunks = 0;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    const int64_t n = (int64_t)f.u * f.u;
+                    if (n == 0) continue;
+                    zs.push_back(ZeroSeg<T>{U(s), n});
+                    start.push_back(chunks);
+                    chunks += (int)((n + 1023) / 1024);
+                }
+                if (!zs.empty())
+                    fsteps.push_back(Step{S_ZERO, farena.add(zs), farena.add(start), (int)zs.size(),
+                                          chunks, false});
+            }
+            for (int slot = 0; slot < 2; ++slot) {
+                std::vector<ExtendDesc<T>> ds;
+                std::vector<int> start;
+                int total = 0;
+                for (int p : lv) {
+                    const FrontLayout& fp = P.fronts[p];
+                    if (fp.nchild <= slot) continue;
+                    const int c = fp.child[slot];
+                    const FrontLayout& fc = P.fronts[c];
+                    if (fc.u == 0) continue;
+                    ds.push_back(ExtendDesc<T>{U(c), P.dRel.as<int>() + fc.rel, L(p), U(p), fc.u,
+                                               fp.n, fp.m, fp.u});
+                    start.push_back(total);
+                    total += fc.u;
+                }
+                if (!ds.empty())
+                    fsteps.push_back(Step{S_EXTEND, farena.add(ds), farena.add(start),
+                                          (int)ds.size(), total, false});
+            }
+            int maxn = 0;
+            for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
+            for (int k = 0; k < maxn; k += NB) {
+                std::vector<DiagDesc<T>> dd;
+                std::vector<GemmDesc<T>> trsm, upd;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    if (f.n <= k) continue;
+                    const int b = std::min(NB, f.n - k);
+                    T* base = L(s);
+                    dd.push_back(DiagDesc<T>{base + k + (int64_t)k * f.m, D(s) + k, Li(s, k), f.m, b,
+                                             s, k});
+                    if (f.m - k - b > 0) {
+                        T* a21 = base + (k + b) + (int64_t)k * f.m;
+                        trsm.push_back(GemmDesc<T>{a21, Li(s, k), a21, nullptr, D(s) + k, f.m, NB,
+                                                   f.m, f.m - k - b, b, b, G_TRANSB | herm});
+                    }
+                    if (f.n - k - b > 0) {
+                        T* a21 = base + (k + b) + (int64_t)k * f.m;
+                        T* a22 = base + (k + b) + (int64_t)(k + b) * f.m;
+                        upd.push_back(GemmDesc<T>{a21, a21, a22, D(s) + k, nullptr, f.m, f.m, f.m,
+                                                  f.m - k - b, f.n - k - b, b,
+                                                  G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                    }
+                }
+                if (!dd.empty())
+                    fsteps.push_back(Step{S_DIAG, farena.add(dd), 0, (int)dd.size(), 0, false});
+                sb.gemm(trsm);
+                sb.gemm(upd);
+            }
+            std::vector<GemmDesc<T>> syrk;
+            for (int s : lv) {
+                const FrontLayout& f = P.fronts[s];
+                if (f.u == 0 || f.n == 0) continue;
+                T* l21 = L(s) + f.n;
+                syrk.push_back(GemmDesc<T>{l21, l21, U(s), D(s), nullptr, f.m, f.m, f.u, f.u, f.u,
+                                           f.n, G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+            }
+            sb.gemm(syrk);
+        }
+        farena.finish();
+    }
+
+    SolveSched& schedule(int k) {
+        auto it = solves.find(k);
+        if (it != solves.end()) return *it->second;
+        auto sp = std::make_unique<SolveSched>();
+        SolveSched& S = *sp;
+        S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
+        S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
+        T* wp = S.wpool.template as<T>();
+        auto Wp = [&](int s) { return wp + P.fronts[s].W * k; };
+        const int cA = P.conjugate ? G_CONJA : 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            const bool bwd = pass == 1;
+            std::vector<Step>& steps = bwd ? S.bwd : S.fwd;
+            StepBuilder<T> sb{S.arena, steps};
+            for (int hh = 0; hh < P.numLevels; ++hh) {
+                const int h = bwd ? P.numLevels - 1 - hh : hh;
+                const auto& lv = P.levels[h];
+                std::vector<SolveSeg> segs;
+                std::vector<int> rs;
+                int rows = 0;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    segs.push_back(SolveSeg{f.W, f.low, f.n, f.m, f.off, 0});
+                    rs.push_back(rows);
+                    rows += f.m;
+                }
+                const size_t segOff = S.arena.add(segs), rsOff = S.arena.add(rs);
+                steps.push_back(Step{S_COPYIN, segOff, rsOff, (int)segs.size(), rows, bwd});
+                int maxn = 0;
+                for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
+                if (!bwd) {
+                    for (int slot = 0; slot < 2; ++slot) {
+                        std::vector<SolveExtend> ds;
+                        std::vector<int> st;
+                        int tot = 0;
+                        for (int p : lv) {
+                            const FrontLayout& fp = P.fronts[p];
+                            if (fp.nchild <= slot) continue;
+                            const FrontLayout& fc = P.fronts[fp.child[slot]];
+                            if (fc.u == 0) continue;
+                            ds.push_back(SolveExtend{fc.W, fp.W, fc.rel, fc.n, fc.m, fc.u, fp.m});
+                            st.push_back(tot);
+                            tot += fc.u;
+                        }
+                        if (!ds.empty())
+                            steps.push_back(Step{S_SEXTEND, S.arena.add(ds), S.arena.add(st),
+                                                 (int)ds.size(), tot, false});
+                    }
+                    for (int k0 = 0; k0 < maxn; k0 += NB) {
+                        std::vector<GemmDesc<T>> tr, up;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.n <= k0) continue;
+                            const int b = std::min(NB, f.n - k0);
+                            T* w = Wp(s);
+                            tr.push_back(GemmDesc<T>{Li(s, k0), w + k0, w + k0, nullptr, nullptr,
+                                                     NB, f.m, f.m, b, k, b, 0});
+                            if (f.m - k0 - b > 0)
+                                up.push_back(GemmDesc<T>{L(s) + (k0 + b) + (int64_t)k0 * f.m, w + k0,
+                                                         w + k0 + b, nullptr, nullptr, f.m, f.m, f.m,
+                                                         f.m - k0 - b, k, b, G_ACCUM | G_NEG});
+                        }
+                        sb.gemm(tr);
+                        sb.gemm(up);
+                    }
+                } else {
+                    std::vector<GemmDesc<T>> g;
+                    for (int s : lv) {
+                        const FrontLayout& f = P.fronts[s];
+                        if (f.u == 0 || f.n == 0) continue;
+                        T* w = Wp(s);
+                        g.push_back(GemmDesc<T>{L(s) + f.n, w + f.n, w, nullptr, nullptr, f.m, f.m,
+                                                f.m, f.n, k, f.u, G_TRANSA | G_ACCUM | G_NEG | cA});
+                    }
+                    sb.gemm(g);
+                    const int maxq = (maxn + NB - 1) / NB;
+                    for (int q = maxq - 1; q >= 0; --q) {
+                        const int k0 = q * NB;
+                        std::vector<GemmDesc<T>> tr, up;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.n <= k0) continue;
+                            const int b = std::min(NB, f.n - k0);
+                            T* w = Wp(s);
+                            tr.push_back(GemmDesc<T>{Li(s, k0), w + k0, w + k0, nullptr, nullptr,
+                                                     NB, f.m, f.m, b, k, b, G_TRANSA | cA});
+                            if (k0 > 0)
+                                up.push_back(GemmDesc<T>{L(s) + k0, w + k0, w, nullptr, nullptr, f.m,
+                                                         f.m, f.m, k0, k, b,
+                                                         G_TRANSA | G_ACCUM | G_NEG | cA});
+                        }
+                        sb.gemm(tr);
+                        sb.gemm(up);
+                    }
+                }
+                steps.push_back(Step{S_COPYOUT, segOff, rsOff, (int)segs.size(), rows, bwd});
+            }
+        }
+        S.arena.finish();
+        SolveSched& ref = S;
+        solves.emplace(k, std::move(sp));
+        return ref;
+    }
+
+    void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
+        hipStream_t st = P.stream;
+        for (const Step& s : steps) {
+            switch (s.kind) {
+                case S_ZERO:
+                    launch_zero_segments<T>(a.at<ZeroSeg<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                            s.nwork, st);
+                    break;
+                case S_EXTEND:
+                    launch_extend_add<T>(a.at<ExtendDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                         s.nwork, st);
+                    break;
+                case S_DIAG:
+                    launch_diag_ldl<T>(a.at<DiagDesc<T>>(s.desc), s.ndesc, P.conjugate,
+                                       P.dStatus.as<int>(), st);
+                    break;
+                case S_GEMM:
+                    launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
+                                   st);
+                    break;
+                case S_COPYIN:
+                    launch_copy_in<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
+                                      S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k, P.dLow.as<int>(),
+                                      s.backward, st);
+                    break;
+                case S_COPYOUT:
+                    launch_copy_out<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc,
+                                       s.nwork, S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k, st);
+                    break;
+                case S_SEXTEND:
+                    launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
+                                           s.nwork, S->wpool.template as<T>(), P.dRel.as<int>(), k, st);
+                    break;
+            }
+        }
+        CQ_HIP(hipGetLastError());
+    }
+
+    void factor(const V* vals) {
+        hipStream_t st = P.stream;
+        CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
+        launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate, st);
+        run(fsteps, farena, nullptr, 0);
+    }
+
+    // x: V, N x k (ld ldx); nodal or original ordering; solved in place
+    void solve(V* x, int64_t ldx, int k, bool nodal) {
+        SolveSched& S = schedule(k);
+        hipStream_t st = P.stream;
+        const int* map = P.dInvMap.as<int>();
+        if (nodal) {
+            if (!identity.p) {
+                std::vector<int> id(P.N);
+                std::iota(id.begin(), id.end(), 0);
+                upload(identity, id);
+            }
+            map = identity.as<int>();
+        }
+        launch_pull<T, V>(x, ldx, map, S.xn.template as<T>(), P.N, k, st);
+        run(S.fwd, S.arena, &S, k);
+        launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k, P.dStatus.as<int>(), st);
+        run(S.bwd, S.arena, &S, k);
+        launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
+        CQ_HIP(hipGetLastError());
+    }
+
+    V* scratch_cols(int k) {
+        if (rbufCols < k) {
+            rbuf.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(V));
+            rbufCols = k;
+        }
+        return rbuf.as<V>();
+    }
+
+    void get_front(int s, void* hL, void* hdiag) {
+        const FrontLayout& f = P.fronts[s];
+        const int64_t mn = (int64_t)f.m * f.n;
+        if (hL && mn > 0) {
+            DevBuf tmp(mn * sizeof(V));
+            launch_widen<T, V>(L(s), tmp.as<V>(), mn, P.stream);
+            CQ_HIP(hipStreamSynchronize(P.stream));
+            CQ_HIP(hipMemcpy(hL, tmp.p, mn * sizeof(V), hipMemcpyDeviceToHost));
+        }
+        if (hdiag && f.n > 0) {
+            CQ_HIP(hipStreamSynchronize(P.stream));
+            CQ_HIP(hipMemcpy(hdiag, D(s), f.n * sizeof(V), hipMemcpyDeviceToHost));
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const int* cols,
+           bool cpx, bool conj, hipStream_t st)
+    : sym(std::move(symIn)), complex(cpx), conjugate(conj), stream(st) {
+    const Symbolic& S = *sym;
+    N = S.N;
+    numFronts = (int)S.nodes.size();
+    nnz = rowOff[N] - rowOff[0];
+    hostRowOff.assign(rowOff, rowOff + N + 1);
+    for (auto& r : hostRowOff) r -= rowOff[0];
+    const int* colp = cols + rowOff[0];
+
+    fronts.resize(numFronts);
+    int64_t Lp = 0, Up = 0, Wp = 0, Ip = 0;
+    std::vector<int> relAll, lowAll;
+    int maxh = 0;
+    for (int s = 0; s < numFronts; ++s) {
+        const SymNode& nd = S.nodes[s];
+        FrontLayout& f = fronts[s];
+        f.n = nd.size;
+        f.u = (int)nd.lowerStruct.size();
+        f.m = f.n + f.u;
+        f.off = nd.off;
+        f.parent = nd.parent;
+        f.nchild = (int)nd.children.size();
+        f.child[0] = f.nchild > 0 ? nd.children[0] : -1;
+        f.child[1] = f.nchild > 1 ? nd.children[1] : -1;
+        f.height = 0;
+        for (int c = 0; c < f.nchild; ++c) f.height = std::max(f.height, fronts[f.child[c]].height + 1);
+        maxh = std::max(maxh, f.height);
+        f.slot = -1;
+        f.L = Lp; Lp += (int64_t)f.m * f.n;
+        f.U = 0;
+        f.W = Wp; Wp += f.m;
+        f.Linv = Ip; Ip += (int64_t)((f.n + NB - 1) / NB) * NB * NB;
+        f.low = (int64_t)lowAll.size();
+        lowAll.insert(lowAll.end(), nd.lowerStruct.begin(), nd.lowerStruct.end());
+    }
+    for (int s = 0; s < numFronts; ++s) {
+        FrontLayout& f = fronts[s];
+        f.rel = (int64_t)relAll.size();
+        if (f.parent >= 0) {
+            const SymNode& p = S.nodes[f.parent];
+            f.slot = (p.children.size() > 0 && p.children[0] == s) ? 0 : 1;
+            const std::vector<int>& r = f.slot == 0 ? p.leftRelInds : p.rightRelInds;
+            if ((int)r.size() != f.u) throw Error(1, "relative indices do not match the update size");
+            relAll.insert(relAll.end(), r.begin(), r.end());
+        } else if (f.u != 0) {
+            throw Error(1, "a root front has a nonempty lower structure");
+        }
+    }
+    factorEntries = Lp;
+    wRows = Wp;
+    linvEntries = Ip;
+    numLevels = numFronts ? maxh + 1 : 0;
+    levels.assign(numLevels, {});
+    for (int s = 0; s < numFronts; ++s) levels[fronts[s].height].push_back(s);
+    // update matrices live from their front's level (extend-add target) to
+    // their parent's level (extend-add source); storage is reused after that
+    {
+        IntervalAlloc ua;
+        for (int h = 0; h < numLevels; ++h) {
+            for (int s : levels[h]) fronts[s].U = ua.alloc((int64_t)fronts[s].u * fronts[s].u);
+            for (int s : levels[h])
+                for (int c = 0; c < fronts[s].nchild; ++c) {
+                    const FrontLayout& fc = fronts[fronts[s].child[c]];
+                    ua.release(fc.U, (int64_t)fc.u * fc.u);
+                }
+        }
+        updateEntries = ua.top;
+        (void)Up;
+    }
+
+    // value -> front scatter map (dist_symm_front_tree/impl.hpp:255-296)
+    std::vector<int64_t> dst(nnz, -1);
+    for (int s = 0; s < numFronts; ++s) {
+        const SymNode& nd = S.nodes[s];
+        const FrontLayout& f = fronts[s];
+        for (int t = 0; t < f.n; ++t) {
+            const int i = nd.inds[t];
+            for (int64_t e = hostRowOff[i]; e < hostRowOff[i + 1]; ++e) {
+                const int c = colp[e];
+                if (c < 0 || c >= N) throw Error(1, "column index out of range");
+                const int target = S.map[c];
+                if (target < f.off + t) continue;
+                int row;
+                if (target < f.off + f.n) {
+                    row = target - f.off;
+                } else {
+                    auto it = std::lower_bound(nd.origLowerStruct.begin(), nd.origLowerStruct.end(),
+                                               target);
+                    if (it == nd.origLowerStruct.end() || *it != target)
+                        throw Error(1, "matrix pattern is not covered by the symbolic analysis");
+                    row = nd.origLowerRelInds[it - nd.origLowerStruct.begin()];
+                }
+                dst[e] = f.L + row + (int64_t)t * f.m;
+            }
+        }
+    }
+
+    if (!stream) {
+        CQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        ownStream = true;
+    }
+    CQ_HIP(hipEventCreate(&ev0_));
+    CQ_HIP(hipEventCreate(&ev1_));
+    upload(dRowOff, hostRowOff);
+    std::vector<int> colv(colp, colp + nnz);
+    upload(dCols, colv);
+    upload(dAsmDst, dst);
+    upload(dRel, relAll);
+    upload(dLow, lowAll);
+    upload(dInvMap, S.inverseMap);
+    dStatus.alloc(4 * sizeof(int));
+    dNorms.alloc(4096 * sizeof(unsigned long long));
+    dVals.alloc(std::max<int64_t>(nnz, 1) * (complex ? 16 : 8));
+}
+
+Plan::~Plan() {
+    num.reset();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (ownStream && stream) (void)hipStreamDestroy(stream);
+}
+
+size_t Plan::device_bytes() const {
+    size_t b = dRowOff.bytes + dCols.bytes + dAsmDst.bytes + dRel.bytes + dLow.bytes +
+               dInvMap.bytes + dStatus.bytes + dNorms.bytes + dVals.bytes;
+    if (num) b += num->device_bytes();
+    return b;
+}
+
+int Plan::check_status(const char* what) {
+    int st[3] = {0, 0, 0};
+    CQ_HIP(hipMemcpyAsync(st, dStatus.p, sizeof(st), hipMemcpyDeviceToHost, stream));
+    CQ_HIP(hipStreamSynchronize(stream));
+    if (st[0] != 0)
+        throw Error(2, std::string(what) + ": zero or non-finite pivot in front " +
+                           std::to_string(st[1]) + " at column " + std::to_string(st[2]));
+    return 0;
+}
+
+template <class F>
+static void dispatch(bool complex, int precision, F&& f) {
+    if (complex) {
+        if (precision == 0) f((cdouble*)nullptr); else f((cfloat*)nullptr);
+    } else {
+        if (precision == 0) f((double*)nullptr); else f((float*)nullptr);
+    }
+}
+
+void Plan::factor(const void* d_vals, int prec, int ftype) {
+    if (prec != 0 && prec != 1) throw Error(1, "unknown precision");
+    if (ftype != 0) throw Error(1, "front type not supported by this build");
+    if (!num || prec != precision) {
+        num.reset();
+        precision = prec;
+        dispatch(complex, prec, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            num = std::make_unique<Numeric<T>>(*this);
+        });
+    }
+    frontType = ftype;
+    factored = false;
+    const size_t vb = complex ? 16 : 8;
+    CQ_HIP(hipMemcpyAsync(dVals.p, d_vals, nnz * vb, hipMemcpyDeviceToDevice, stream));
+    CQ_HIP(hipMemsetAsync(dStatus.p, 0, dStatus.bytes, stream));
+    CQ_HIP(hipMemsetAsync(dNorms.p, 0, sizeof(unsigned long long), stream));
+    if (complex)
+        launch_row_abs_sum_max<cdouble>(N, dRowOff.as<int64_t>(), dVals.as<cdouble>(),
+                                        dNorms.as<unsigned long long>(), stream);
+    else
+        launch_row_abs_sum_max<double>(N, dRowOff.as<int64_t>(), dVals.as<double>(),
+                                       dNorms.as<unsigned long long>(), stream);
+    CQ_HIP(hipEventRecord(ev0_, stream));
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        static_cast<Numeric<T>*>(num.get())->factor(dVals.as<V>());
+    });
+    CQ_HIP(hipEventRecord(ev1_, stream));
+    unsigned long long na = 0;
+    CQ_HIP(hipMemcpyAsync(&na, dNorms.p, sizeof(na), hipMemcpyDeviceToHost, stream));
+    check_status("LDL");
+    float ms = 0.f;
+    CQ_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+    lastFactorMs_ = ms;
+    double nd;
+    std::memcpy(&nd, &na, sizeof(nd));
+    normA = nd;
+    factored = true;
+}
+
+void Plan::solve(void* d_x, int64_t ldx, int k, bool nodal) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (k <= 0) return;
+    if (ldx < N) throw Error(1, "leading dimension smaller than N");
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        static_cast<Numeric<T>*>(num.get())->solve(static_cast<V*>(d_x), ldx, k, nodal);
+    });
+    check_status("Solve");
+}
+
+void Plan::multiply(double alpha, const void* d_x, int64_t ldx, double beta, void* d_y,
+                    int64_t ldy, int k) {
+    if (!factored) throw Error(6, "the plan holds no matrix values yet");
+    if (complex)
+        launch_spmv<cdouble>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<cdouble>(), alpha,
+                             (const cdouble*)d_x, ldx, beta, (cdouble*)d_y, ldy, k, stream);
+    else
+        launch_spmv<double>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<double>(), alpha,
+                            (const double*)d_x, ldx, beta, (double*)d_y, ldy, k, stream);
+    CQ_HIP(hipGetLastError());
+}
+
+void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, double tol,
+                  int maxIters, int* itersOut, int* convOut, double* etaOut) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (k <= 0) return;
+    if (k > 1000) throw Error(1, "at most 1000 right-hand sides per refinement call");
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        auto* nm = static_cast<Numeric<T>*>(num.get());
+        const V* b = static_cast<const V*>(d_b);
+        V* x = static_cast<V*>(d_x);
+        V* r = nm->scratch_cols(k);
+        unsigned long long* nrm = dNorms.as<unsigned long long>();
+        // nrm[0] = ||A||, nrm[1..k] = |b|, [k+1..2k] = |x|, [2k+1..3k] = |r|
+        CQ_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long) * 3 * k, stream));
+        launch_col_abs_max<V>(b, ldb, N, k, nrm + 1, stream);
+        launch_copy_cols<V>(x, ldx, b, ldb, N, k, stream);
+        nm->solve(x, ldx, k, false);
+        std::vector<unsigned long long> h(3 * k + 1);
+        int it = 0;
+        bool conv = false;
+        double eta = 0.0;
+        for (;;) {
+            launch_copy_cols<V>(r, N, b, ldb, N, k, stream);
+            launch_spmv<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<V>(), -1.0, x, ldx,
+                           1.0, r, N, k, stream);
+            CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
+            launch_col_abs_max<V>(x, ldx, N, k, nrm + 1 + k, stream);
+            launch_col_abs_max<V>(r, N, N, k, nrm + 1 + 2 * k, stream);
+            CQ_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(unsigned long long) * (3 * k + 1),
+                                  hipMemcpyDeviceToHost, stream));
+            CQ_HIP(hipStreamSynchronize(stream));
+            eta = 0.0;
+            for (int j = 0; j < k; ++j) {
+                double bn, xn, rn;
+                std::memcpy(&bn, &h[1 + j], 8);
+                std::memcpy(&xn, &h[1 + k + j], 8);
+                std::memcpy(&rn, &h[1 + 2 * k + j], 8);
+                const double den = normA * xn + bn;
+                const double e = den > 0 ? rn / den : 0.0;
+                eta = std::max(eta, std::isfinite(e) ? e : INFINITY);
+            }
+            if (eta <= tol) { conv = true; break; }
+            if (it >= maxIters) break;
+            nm->solve(r, N, k, false);
+            launch_axpy_cols<V>(x, ldx, r, N, N, k, stream);
+            ++it;
+        }
+        *itersOut = it;
+        *convOut = conv ? 1 : 0;
+        *etaOut = eta;
+    });
+    check_status("refine");
+}
+
+void Plan::adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, int k,
+                   void* d_lambda, int64_t ldl, void* d_dA) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (k <= 0) return;
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        auto* nm = static_cast<Numeric<T>*>(num.get());
+        V* lam = static_cast<V*>(d_lambda);
+        launch_copy_cols<V>(lam, ldl, static_cast<const V*>(d_g), ldg, N, k, stream);
+        nm->solve(lam, ldl, k, false);
+        if (d_dA)
+            launch_adjoint_dA<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), lam, ldl,
+                                 static_cast<const V*>(d_x), ldx, k, conjugate,
+                                 static_cast<V*>(d_dA), stream);
+    });
+    check_status("adjoint");
+}
+
+void Plan::get_front(int s, void* hL, void* hdiag) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (s < 0 || s >= numFronts) throw Error(1, "front index out of range");
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        static_cast<Numeric<T>*>(num.get())->get_front(s, hL, hdiag);
+    });
+}
+
+}  // namespace cliqg

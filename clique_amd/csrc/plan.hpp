@@ -1,0 +1,93 @@
+// Device plan: front layout in HBM, static level schedule, numeric drivers.
+#pragma once
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "kernels.hpp"
+#include "symbolic.hpp"
+
+namespace cliqg {
+
+// Owning device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    void alloc(size_t n);
+    void release();
+    template <class X> X* as() const { return static_cast<X*>(p); }
+};
+
+// Per-front layout (host copy).
+struct FrontLayout {
+    int n, u, m, off;
+    int parent, slot, height, nchild;
+    int child[2];
+    int64_t L;      // factor pool offset (elements)
+    int64_t U;      // update pool offset
+    int64_t rel;    // offset of relInds into the parent's front (length u)
+    int64_t low;    // offset of lowerStruct (length u)
+    int64_t W;      // solve workspace row offset (prefix of m)
+    int64_t Linv;   // inverse diagonal block pool offset
+};
+
+// Opaque per-working-type numeric state.
+struct NumericBase {
+    virtual ~NumericBase() = default;
+    virtual size_t device_bytes() const = 0;
+};
+
+class Plan {
+public:
+    Plan(std::shared_ptr<const Symbolic> sym, const int64_t* rowOff, const int* cols,
+         bool complex, bool conjugate, hipStream_t stream);
+    ~Plan();
+
+    void factor(const void* d_vals, int precision, int frontType);
+    void solve(void* d_x, int64_t ldx, int k, bool nodal);
+    void multiply(double alpha, const void* d_x, int64_t ldx, double beta, void* d_y,
+                  int64_t ldy, int k);
+    void refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, double tol,
+                int maxIters, int* iters, int* converged, double* maxEta);
+    void adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, int k,
+                 void* d_lambda, int64_t ldl, void* d_dA);
+    void get_front(int s, void* h_L, void* h_diag);
+    size_t device_bytes() const;
+    double last_factor_ms() const { return lastFactorMs_; }
+
+    // shared layout, read by the numeric engines
+    std::shared_ptr<const Symbolic> sym;
+    bool complex, conjugate;
+    int N = 0, numFronts = 0, numLevels = 0;
+    int64_t nnz = 0;
+    std::vector<FrontLayout> fronts;
+    std::vector<std::vector<int>> levels;   // fronts of each height
+    int64_t factorEntries = 0, updateEntries = 0, linvEntries = 0, wRows = 0;
+    std::vector<int64_t> hostRowOff;
+    hipStream_t stream = nullptr;
+    bool ownStream = false;
+
+    DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dInvMap, dVals, dStatus, dNorms;
+    int precision = -1, frontType = 0;
+    bool factored = false;
+    double normA = 0.0;
+
+    std::unique_ptr<NumericBase> num;
+    int check_status(const char* what);
+
+private:
+    double lastFactorMs_ = 0.0;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+};
+
+}  // namespace cliqg

@@ -1,0 +1,180 @@
+/*
+ * cliqg.h -- C ABI of the MI355X multifrontal LDL^T engine (libcliqg.so).
+ *
+ * Clique exposes its hot path only as header-only C++ templates in
+ * namespace cliq (include/clique.hpp:19-90); there is no FFI. This ABI is
+ * the boundary those templates (and any other host language) bind to: plain
+ * pointers, sizes and integer status codes, no C++ or HIP types. The
+ * header-only C++ surface in include/clique.hpp is written on top of it.
+ *
+ * Conventions
+ *  - Every function returns a cliqg_status; CLIQG_OK is 0. The message of the
+ *    last failure on the calling thread is returned by cliqg_last_error().
+ *  - "d_" arguments are device pointers (HBM of the plan's device);
+ *    "h_" arguments are host pointers.
+ *  - Matrices are column-major with an explicit leading dimension, as
+ *    El::Matrix is. Complex values are interleaved (re, im) doubles or
+ *    floats, the layout of std::complex.
+ *  - Indices and orderings: "original" is the user's ordering of the
+ *    unknowns; "nodal" is the nested-dissection ordering in which front s
+ *    owns the contiguous rows [off_s, off_s + size_s) (DistNodalMultiVec,
+ *    include/clique/numeric/dist_nodal_multi_vec/decl.hpp:19-51).
+ *  - One plan serves one device and one stream. Calls on one plan must not
+ *    overlap (the reference's Solve is likewise not reentrant on one tree:
+ *    it writes the front tree's mutable work matrices,
+ *    include/clique/numeric/dist_symm_front_tree/decl.hpp:183, 202-203).
+ */
+#ifndef CLIQG_H
+#define CLIQG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    CLIQG_OK = 0,
+    CLIQG_ERR_INVALID = 1,       /* bad argument / API misuse (El::LogicError) */
+    CLIQG_ERR_SINGULAR = 2,      /* zero or non-finite pivot                   */
+    CLIQG_ERR_NONCONVERGED = 3,  /* refinement did not reach the tolerance     */
+    CLIQG_ERR_OOM = 4,           /* device or host allocation failed           */
+    CLIQG_ERR_HIP = 5,           /* HIP runtime error                          */
+    CLIQG_ERR_STATE = 6          /* wrong plan state (e.g. already factored)   */
+} cliqg_status;
+
+typedef enum { CLIQG_REAL = 0, CLIQG_COMPLEX = 1 } cliqg_field;
+typedef enum { CLIQG_F64 = 0, CLIQG_F32 = 1 } cliqg_precision;
+
+/* SymmFrontType of the factored tree
+ * (include/clique/numeric/dist_symm_front_tree/decl.hpp:17-47). 1D/2D
+ * distinctions are layout choices for P>1 and collapse at world size 1. */
+typedef enum {
+    CLIQG_FRONT_LDL = 0,        /* LDL_1D / LDL_2D: unit L, diagonal D          */
+    CLIQG_FRONT_BLOCK_LDL = 1   /* BLOCK_LDL_*: ATL := inverse, ABL original A21 */
+} cliqg_front_type;
+
+/* which-array selectors for cliqg_symbolic_node_array */
+typedef enum {
+    CLIQG_NODE_INDS = 0,            /* original indices of the node (SepOrLeaf::inds) */
+    CLIQG_NODE_LOWER_STRUCT = 1,    /* SymmNodeInfo::lowerStruct                      */
+    CLIQG_NODE_ORIG_LOWER_STRUCT = 2,
+    CLIQG_NODE_ORIG_LOWER_REL = 3,  /* origLowerRelInds                               */
+    CLIQG_NODE_LEFT_REL = 4,        /* leftRelInds                                    */
+    CLIQG_NODE_RIGHT_REL = 5        /* rightRelInds                                   */
+} cliqg_node_array;
+
+typedef struct cliqg_symbolic_s* cliqg_symbolic;
+typedef struct cliqg_plan_s* cliqg_plan;
+
+/* Per-RHS outcome of iterative refinement. */
+typedef struct {
+    int iterations;          /* refinement steps taken (0 = the first solve sufficed) */
+    int converged;           /* all columns reached tol                               */
+    double max_backward_error;  /* max over columns of ||b-Ax||_inf/(||A||_inf||x||_inf+||b||_inf) */
+} cliqg_refine_info;
+
+/* ---- library ------------------------------------------------------------ */
+const char* cliqg_last_error(void);
+cliqg_status cliqg_version(int* major, int* minor);
+/* Device-memory helpers so hosts need no HIP toolchain of their own.
+ * kind: 0 host->device, 1 device->host, 2 device->device. */
+cliqg_status cliqg_device_malloc(void** d_ptr, size_t bytes);
+cliqg_status cliqg_device_free(void* d_ptr);
+cliqg_status cliqg_memcpy(void* dst, const void* src, size_t bytes, int kind);
+cliqg_status cliqg_device_synchronize(void);
+
+/* ---- symbolic analysis (host) --------------------------------------------
+ * Replaces NaturalNestedDissection (include/clique/symbolic/
+ * natural_nested_dissection.hpp:17-26, 447-488) at world size 1: separator
+ * tree, reordering map and DistSymmInfo (dist_symm_info.hpp:17-113). The
+ * graph is the CSR adjacency of A's pattern in the original ordering (the
+ * reference uses A.DistGraph(), tests/Solve.cpp:123-131). */
+cliqg_status cliqg_symbolic_natural(int nx, int ny, int nz, int N,
+                                    const int64_t* h_row_off, const int* h_cols,
+                                    int cutoff, cliqg_symbolic* out);
+/* A forest of independent problems: symbolic i occupies nodal and original
+ * indices shifted by the sizes of symbolics 0..i-1 (batched solves of many
+ * unrelated systems in one schedule; no reference counterpart). */
+cliqg_status cliqg_symbolic_concat(const cliqg_symbolic* syms, int count, cliqg_symbolic* out);
+cliqg_status cliqg_symbolic_destroy(cliqg_symbolic sym);
+cliqg_status cliqg_symbolic_num_nodes(cliqg_symbolic sym, int* num_nodes, int* N);
+/* info[0..7] = size, off, parent, child0, child1, |lowerStruct|,
+ *              |origLowerStruct|, onLeft   (SymmNodeInfo fields) */
+cliqg_status cliqg_symbolic_node(cliqg_symbolic sym, int s, int* h_info);
+/* copies the selected array (length from cliqg_symbolic_node) into h_out */
+cliqg_status cliqg_symbolic_node_array(cliqg_symbolic sym, int s, int which, int* h_out);
+/* map[original] = nodal (DistMap returned by NaturalNestedDissection) */
+cliqg_status cliqg_symbolic_map(cliqg_symbolic sym, int* h_map);
+/* FactorizationWork / SolveWork (one sweep, one RHS) and sum of m*n
+ * (include/clique/numeric/dist_symm_front_tree/impl.hpp:540-639). */
+cliqg_status cliqg_symbolic_work(cliqg_symbolic sym, int is_complex, double* factor_flops,
+                                 double* solve_flops_per_rhs, double* factor_entries);
+
+/* ---- plan: device front tree --------------------------------------------
+ * Replaces the DistSymmFrontTree constructor (dist_symm_front_tree/
+ * decl.hpp:216-221, impl.hpp:22-384): records A's pattern (CSR in the
+ * original ordering, rows sorted and de-duplicated), builds the
+ * value-to-front scatter map and the static level schedule, and uploads
+ * them. `conjugate` selects Hermitian (LDL^H) instead of symmetric.
+ * stream may be NULL (the plan creates its own). */
+cliqg_status cliqg_plan_create(cliqg_symbolic sym, const int64_t* h_row_off, const int* h_cols,
+                               cliqg_field field, int conjugate, void* hip_stream,
+                               cliqg_plan* out);
+cliqg_status cliqg_plan_destroy(cliqg_plan plan);
+/* bytes of device memory the plan holds now */
+cliqg_status cliqg_plan_memory(cliqg_plan plan, size_t* bytes);
+
+/* Numeric factorization. Replaces DistSymmFrontTree::Initialize's scatter of
+ * A into the fronts (impl.hpp:241-297) followed by cliq::LDL (ldl.hpp:57-75;
+ * LocalLDL ldl/local.hpp:25-122, FrontLDL ldl/local_front.hpp:29-71,
+ * FrontBlockLDL ldl/local_front_block.hpp:25-73). d_vals holds A's nnz values
+ * (double or interleaved complex double) in the CSR order given at plan
+ * creation. precision F32 factors in (complex) FP32 with FP64 pivots D.
+ * Refactoring with new values is allowed (no "already factored" error: the
+ * pattern is unchanged, the values are re-scattered). */
+cliqg_status cliqg_plan_factor(cliqg_plan plan, const void* d_vals, cliqg_precision precision,
+                               cliqg_front_type front_type);
+
+/* Solve with the factor, in place. Replaces DistNodalMultiVec::Pull
+ * (dist_nodal_multi_vec/impl.hpp:67-217), cliq::Solve (solve.hpp:40-92) and
+ * Push (impl.hpp:219-356). d_x is N x k, column-major (ld = ldx), FP64 or
+ * complex FP64, original ordering. */
+cliqg_status cliqg_plan_solve(cliqg_plan plan, void* d_x, int ldx, int k);
+/* The same on a nodal-ordered right-hand side (no Pull/Push). */
+cliqg_status cliqg_plan_solve_nodal(cliqg_plan plan, void* d_x, int ldx, int k);
+
+/* y := alpha A x + beta y with the factored values, FP64 (replaces
+ * cliq::Multiply, numeric/multiply.hpp:27-166, at world size 1). */
+cliqg_status cliqg_plan_multiply(cliqg_plan plan, double alpha, const void* d_x, int ldx,
+                                 double beta, void* d_y, int ldy, int k);
+
+/* FP64 iterative refinement (no reference counterpart): x := solve(b); while
+ * some column has backward error > tol and iterations < max_iters:
+ * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified. */
+cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* d_x, int ldx,
+                               int k, double tol, int max_iters, cliqg_refine_info* info);
+
+/* Adjoint of x = A^{-1} b (no reference counterpart). Given g = dL/dx and the
+ * solution x (N x k, original ordering): lambda := A^{-T} g (A^{-H} g when
+ * the plan is Hermitian) with the same factor, dL/db = lambda written to
+ * d_lambda, and dL/dA on A's pattern, one value per stored nonzero
+ * (i, j): dA[e] = -sum_r lambda(i, r) * x(j, r)   (conj(x) when Hermitian).
+ * Stored entries are treated as independent parameters. */
+cliqg_status cliqg_plan_adjoint(cliqg_plan plan, const void* d_g, int ldg, const void* d_x,
+                                int ldx, int k, void* d_lambda, int ldl, void* d_dA);
+
+/* Lazy host mirror of front s after factorization: h_L gets the m x n
+ * factor (column-major, FP64/complex FP64 regardless of the factor
+ * precision), h_diag the n pivots (LDL fronts only; may be NULL). Replaces
+ * reading frontTree.localFronts[s].frontL/diag (tests/Refactor.cpp:157-160). */
+cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_diag);
+
+/* Timing of the last factorization on the plan's stream (HIP events). */
+cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

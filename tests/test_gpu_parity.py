@@ -1,0 +1,183 @@
+"""GPU parity of the factorization and solves against the CPU oracle.
+
+The oracle restates Clique's FP64 LocalLDL/FrontLDL (nb = 96, the blocksize
+of tests/Solve.cpp:39) and its solves; the GPU path uses 64-wide diagonal
+blocks and one delayed Schur update per front, so FP64 results agree to
+rounding, not bitwise. Tolerances are stated per test.
+"""
+import numpy as np
+import pytest
+
+import clique_amd as cq
+import oracle
+from helpers import backward_error, hermitian_matrix, problem, rel_diff
+
+torch = pytest.importorskip("torch")
+
+
+def gpu_tree(ip, ind, v, dims, cutoff, is_complex=False, conjugate=False, precision="f64",
+             front_type="ldl"):
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=cutoff)
+    ft = cq.DistSymmFrontTree(sym, ip, ind, is_complex=is_complex, conjugate=conjugate)
+    ft.factor(torch.tensor(v, device="cuda:0"), precision=precision, front_type=front_type)
+    return sym, ft
+
+
+def oracle_tree(ip, ind, v, dims, cutoff, is_complex=False, conjugate=False, block=False):
+    t = oracle.Tree(*dims, ip, ind, cutoff=cutoff)
+    f = oracle.Factor(t, is_complex=is_complex, conjugate=conjugate)
+    f.initialize(ip, ind, v)
+    f.ldl(nb=96, block=block)
+    return t, f
+
+
+def to_dev(a):
+    return cq.colmajor(torch.tensor(np.asarray(a), device="cuda:0"))
+
+
+CASES = [
+    ("box", (2, 1, 1), 1),
+    ("box", (7, 5, 3), 4),
+    ("box", (17, 1, 1), 2),
+    ("box", (12, 9, 1), 10),
+    ("lap", 6, 8),
+    ("lap", 11, 16),
+    ("lap", 30, 128),      # config 1 (N = 27,000, 363 fronts)
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,cut", CASES)
+def test_fronts_match_oracle_fp64(device, kind, n, cut):
+    ip, ind, v, dims = problem(kind, n)
+    sym, ft = gpu_tree(ip, ind, v, dims, cut)
+    t, f = oracle_tree(ip, ind, v, dims, cut)
+    worst = 0.0
+    for s in range(sym.num_nodes):
+        Lg, dg = ft.front(s)
+        Lo, do = f.front(s)
+        scale = max(np.abs(Lo).max(initial=0), 1.0)
+        worst = max(worst, np.abs(Lg - Lo).max(initial=0) / scale)
+        if do.size:
+            assert np.abs(dg - do).max() <= 1e-12 * np.abs(do).max()
+    assert worst <= 1e-12, worst
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,cut", CASES)
+@pytest.mark.parametrize("k", [1, 5])
+def test_solve_matches_oracle_fp64(device, kind, n, cut, k):
+    ip, ind, v, dims = problem(kind, n)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut)
+    t, f = oracle_tree(ip, ind, v, dims, cut)
+    b = np.random.default_rng(1).standard_normal((N, k))
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    xo = f.solve(b)
+    assert rel_diff(xg, xo) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cut", [(8, 16), (16, 64)])
+def test_complex_symmetric_helmholtz_fp64(device, n, cut):
+    ip, ind, v, dims = problem("helm", n)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut, is_complex=True)
+    t, f = oracle_tree(ip, ind, v, dims, cut, is_complex=True)
+    for s in range(sym.num_nodes):
+        Lg, dg = ft.front(s)
+        Lo, do = f.front(s)
+        assert np.abs(Lg - Lo).max(initial=0) <= 1e-11 * max(np.abs(Lo).max(initial=0), 1)
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    assert rel_diff(xg, f.solve(b)) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-13
+
+
+@pytest.mark.gpu
+def test_hermitian_ldlh_fp64(device):
+    ip, ind, v, dims = hermitian_matrix(6)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 16, is_complex=True, conjugate=True)
+    t, f = oracle_tree(ip, ind, v, dims, 16, is_complex=True, conjugate=True)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal((N, 2)) + 1j * rng.standard_normal((N, 2))
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    assert rel_diff(xg, f.solve(b)) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,cut,cplx", [("lap", 30, 128, False), ("helm", 16, 64, True),
+                                             ("pml", 16, 64, True)])
+def test_fp32_factor_with_fp64_refinement(device, kind, n, cut, cplx):
+    ip, ind, v, dims = problem(kind, n)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut, is_complex=cplx, precision="f32")
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    bd = to_dev(b)
+    x, info = ft.refine(bd, tol=1e-10, max_iters=30)
+    xg = x.cpu().numpy()
+    eta = backward_error(ip, ind, v, xg, b)
+    assert info["converged"] and eta.max() <= 1e-10, (info, eta)
+    assert abs(info["backward_error"] - eta.max()) <= 1e-2 * eta.max() + 1e-16
+
+
+@pytest.mark.gpu
+def test_multiply_matches_oracle(device):
+    ip, ind, v, dims = problem("helm", 10)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 32, is_complex=True)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    y0 = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    yd = to_dev(y0)
+    ft.multiply(to_dev(x), yd, alpha=-1.0, beta=2.0)
+    yo = oracle.multiply(ip, ind, v, x, alpha=-1.0, beta=2.0, y=y0)
+    assert rel_diff(yd.cpu().numpy(), yo) <= 1e-14
+
+
+@pytest.mark.gpu
+def test_forest_of_independent_problems(device):
+    """Concatenated symbolics factor as one forest; each block matches its own solve."""
+    import scipy.sparse as sp
+    shapes = [(9, 7, 1), (5, 5, 5), (12, 3, 1)]
+    syms, mats, rhs = [], [], []
+    for i, sh in enumerate(shapes):
+        ip, ind, v, dims = problem("box", sh, diag=4.3 + i)
+        syms.append(cq.NaturalNestedDissection(*dims, ip, ind, cutoff=8))
+        mats.append(sp.csr_matrix((v, ind, ip)))
+    A = sp.block_diag(mats, format="csr")
+    A.sort_indices()
+    sym = cq.Symbolic.concat(syms)
+    ft = cq.DistSymmFrontTree(sym, A.indptr, A.indices)
+    ft.factor(torch.tensor(A.data, device="cuda:0"))
+    b = np.random.default_rng(6).standard_normal((A.shape[0], 2))
+    x = to_dev(b)
+    ft.solve(x)
+    eta = backward_error(A.indptr, A.indices, A.data, x.cpu().numpy(), b)
+    assert eta.max() <= 1e-14
+
+
+@pytest.mark.gpu
+def test_zero_pivot_is_reported(device):
+    ip, ind, v, dims = problem("box", (4, 4, 1), diag=4.0)
+    v = v.copy()
+    N = ip.size - 1
+    # a zero row/column makes the first pivot of its front exactly zero
+    rows = np.repeat(np.arange(N), np.diff(ip))
+    v[(rows == 0) | (ind == 0)] = 0.0
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=4)
+    ft = cq.DistSymmFrontTree(sym, ip, ind)
+    with pytest.raises(cq.CliqgError) as e:
+        ft.factor(torch.tensor(v, device="cuda:0"))
+    assert e.value.code == 2
