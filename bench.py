@@ -1,0 +1,244 @@
+"""Benchmark: multifrontal LDL^T factor TFLOP/s and time-to-1e-10 backward error,
+FP32 factor + FP64 iterative refinement versus the plain FP64 path, on one GPU.
+
+One step = factor A in (complex) FP32 on the GPU and refine the k right-hand
+sides in FP64 until every column has backward error
+||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= 1e-10 (inputs resident
+in HBM). The same number of plain-FP64 steps (factor + solve, refined if a
+column misses 1e-10) is timed after the FP32 steps. Factor flops follow
+Clique's FactorizationWork convention (complex counted x4).
+
+Default workload: 120^3 complex PML Helmholtz (7-point, 10-cell PML,
+s = 1 + 20i (d/L)^2, omega = 2 pi 10), natural nested dissection with cutoff
+128, 64 unit point sources at seeded interior nodes.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LDL^T factor TFLOP/s & time-to-1e-10 backward error, FP32+IR vs FP64, 1 GPU"
+PEAK = {"fp32": 157.3, "fp64": 78.6}   # MI355X dense MFMA TFLOP/s
+HBM_PEAK = 8000.0                      # GB/s
+
+
+def workload(cfg, n_override=None):
+    from clique_amd import problems as P
+    rng = np.random.default_rng(2024)
+    if cfg == 1:
+        n = n_override or 30
+        ip, ind, v, dims = P.laplacian7(n)
+        N = ip.size - 1
+        b = rng.standard_normal((N, 1))
+        desc = f"{n}^3 real 7-point Laplacian (diag 6.05), natural ND cutoff 128, 1 RHS"
+        return ip, ind, v, dims, b, False, desc
+    if cfg == 2:
+        n = n_override or 64
+        ip, ind, v, dims = P.helmholtz7(n)
+        N = ip.size - 1
+        b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+        desc = f"{n}^3 complex-symmetric Helmholtz (omega 2pi*6), natural ND cutoff 128, 4 RHS"
+        return ip, ind, v, dims, b, True, desc
+    if cfg == 4:
+        n = n_override or 80
+        ip, ind, v, dims = P.diffusion_grf(n, seed=0)
+        N = ip.size - 1
+        b = rng.standard_normal((N, 8))
+        desc = f"{n}^3 GRF diffusion (7-point, kappa=exp(g)), natural ND cutoff 128, 8 RHS"
+        return ip, ind, v, dims, b, False, desc
+    n = n_override or 120
+    ip, ind, v, dims = P.helmholtz_pml(n, omega=2 * np.pi * 10, pml=10, strength=20.0)
+    N = ip.size - 1
+    k = 64
+    # interior nodes: at least the PML width from every face
+    pts = []
+    while len(pts) < k:
+        x, y, z = rng.integers(10, n - 10, size=3)
+        idx = int(x + n * (y + n * z))
+        if idx not in pts:
+            pts.append(idx)
+    b = np.zeros((N, k), dtype=np.complex128)
+    b[pts, np.arange(k)] = 1.0
+    desc = (f"{n}^3 complex PML Helmholtz (7-point, 10-cell PML, s=1+20i(d/L)^2, omega 2pi*10), "
+            f"natural ND cutoff 128, {k} point-source RHS")
+    return ip, ind, v, dims, b, True, desc
+
+
+def cpu_baseline(budget_s=20.0):
+    """Oracle FP64 LDL (the reference's CPU algorithm restated in C, OpenMP)
+    on a bounded sample of the same operator family; TFLOP/s."""
+    import oracle
+    from clique_amd import problems as P
+    threads = min(16, len(os.sched_getaffinity(0)))
+    oracle.build()
+    n = 24
+    best = None
+    while True:
+        ip, ind, v, dims = P.helmholtz_pml(n, omega=2 * np.pi * 10, pml=max(2, n // 12))
+        t = oracle.Tree(*dims, ip, ind, cutoff=128)
+        f = oracle.Factor(t, is_complex=True)
+        f.initialize(ip, ind, v)
+        t0 = time.perf_counter()
+        f.ldl(nb=96, nthreads=threads)
+        dt = time.perf_counter() - t0
+        flops = t.work(True)[0]
+        best = (n, flops, dt)
+        # grow until one run takes a good fraction of the budget
+        nxt = int(n * 1.25)
+        if dt * (nxt / n) ** 6 > budget_s or n >= 120:
+            break
+        n = nxt
+    n, flops, dt = best
+    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle FP64 complex LDL (FrontLDL nb=96, OpenMP) of the {n}^3 PML "
+                       f"Helmholtz operator: {flops / 1e9:.1f} GFLOP in {dt:.2f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=5, choices=[1, 2, 4, 5])
+    ap.add_argument("--n", type=int, default=None, help="override the grid size")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import clique_amd as cq
+
+    t_setup = time.perf_counter()
+    ip, ind, v, dims, b, cplx, desc = workload(args.config, args.n)
+    N = ip.size - 1
+    k = b.shape[1]
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
+    flops, solve_flops, entries = sym.work(cplx)
+    dev = torch.device("cuda:0")
+    vals = torch.tensor(v, device=dev)
+    bd = cq.colmajor(torch.tensor(b, device=dev))
+    x32 = cq.colmajor(torch.empty_like(bd))
+    print(f"# {desc}: N={N} fronts={sym.num_nodes} factor {flops / 1e9:.1f} GFLOP, "
+          f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+
+    p32 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+
+    def step32():
+        p32.factor(vals, precision="f32")
+        fms = p32.last_factor_ms()
+        _, info = p32.refine(bd, x32, tol=args.tol, max_iters=30)
+        return fms, info
+
+    for _ in range(args.warmup):
+        step32()
+    torch.cuda.synchronize()
+    f32_ms, infos = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fms, info = step32()
+        f32_ms.append(fms)
+        infos.append(info)
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / args.steps
+    print(f"# fp32+IR: {t32 * 1e3:.1f} ms/step, factor {np.mean(f32_ms):.1f} ms, "
+          f"IR iterations {[i['iterations'] for i in infos]}, eta {infos[-1]['backward_error']:.2e}",
+          file=sys.stderr, flush=True)
+
+    # profiled (untimed) FP32 factorization: per-kernel-family time and flops
+    p32.set_profiling(True)
+    p32.factor(vals, precision="f32")
+    prof32 = p32.profile()
+    p32.set_profiling(False)
+    mem32 = p32.memory()
+    del p32
+    torch.cuda.empty_cache()
+
+    fp64 = None
+    if not args.no_fp64:
+        p64 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+        x64 = cq.colmajor(torch.empty_like(bd))
+
+        def step64():
+            p64.factor(vals, precision="f64")
+            fms = p64.last_factor_ms()
+            _, info = p64.refine(bd, x64, tol=args.tol, max_iters=5)
+            return fms, info
+
+        for _ in range(args.warmup):
+            step64()
+        torch.cuda.synchronize()
+        f64_ms, infos64 = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fms, info = step64()
+            f64_ms.append(fms)
+            infos64.append(info)
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) / args.steps
+        p64.set_profiling(True)
+        p64.factor(vals, precision="f64")
+        prof64 = p64.profile()
+        fp64 = {"time_to_tol_ms": t64 * 1e3, "factor_ms": float(np.mean(f64_ms)),
+                "factor_tflops": flops / (np.mean(f64_ms) * 1e-3) / 1e12,
+                "refine_iterations": infos64[-1]["iterations"],
+                "backward_error": infos64[-1]["backward_error"],
+                "gemm_tflops": prof64["gemm"]["flops"] / (prof64["gemm"]["ms"] * 1e-3) / 1e12}
+        print(f"# fp64: {t64 * 1e3:.1f} ms/step, factor {np.mean(f64_ms):.1f} ms",
+              file=sys.stderr, flush=True)
+        del p64
+
+    g = prof32["gemm"]
+    achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
+    breakdown = {name: {"launches": e["launches"], "ms": round(e["ms"], 3),
+                        "tflops": (e["flops"] / (e["ms"] * 1e-3) / 1e12) if e["flops"] else None,
+                        "gbps": (e["bytes"] / (e["ms"] * 1e-3) / 1e9) if e["bytes"] else None}
+                 for name, e in prof32.items()}
+    fac32 = float(np.mean(f32_ms))
+    out = {
+        "metric": METRIC,
+        "value": flops / (fac32 * 1e-3) / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "c32" if cplx else "f32",
+        "data": "synthetic",
+        "config": {"workload": desc, "N": N, "fronts": sym.num_nodes, "rhs": k,
+                   "factor_gflop": flops / 1e9, "factor_entries": entries,
+                   "parallelism": "single GPU"},
+        "fp32_ir": {"time_to_tol_ms": t32 * 1e3, "factor_ms": fac32,
+                    "refine_iterations": infos[-1]["iterations"],
+                    "backward_error": infos[-1]["backward_error"], "tol": args.tol,
+                    "device_bytes": mem32},
+        "fp64": fp64,
+        "speedup_fp32ir_vs_fp64": (fp64["time_to_tol_ms"] / (t32 * 1e3)) if fp64 else None,
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (grouped MFMA GEMM, FP32 factor)",
+                     "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "launches": g["launches"], "avg_launch_ms": g["ms"] / g["launches"],
+                     "flops_per_launch": g["flops"] / g["launches"]},
+        "breakdown_fp32_factor": breakdown,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

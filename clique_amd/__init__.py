@@ -255,6 +255,18 @@ class DistSymmFrontTree:
         _lib.check(lib().cliqg_plan_memory(self._h, ctypes.byref(b)))
         return b.value
 
+    def set_profiling(self, enable=True):
+        _lib.check(lib().cliqg_plan_set_profiling(self._h, int(enable)))
+
+    def profile(self):
+        """{kernel family: {launches, ms, flops, bytes}} since profiling was enabled."""
+        n = ctypes.c_int()
+        _lib.check(lib().cliqg_plan_get_profile(self._h, None, 0, ctypes.byref(n)))
+        arr = (_lib.ProfileEntry * max(n.value, 1))()
+        _lib.check(lib().cliqg_plan_get_profile(self._h, arr, n.value, ctypes.byref(n)))
+        return {e.name.decode(): {"launches": e.launches, "ms": e.total_ms, "flops": e.flops,
+                                  "bytes": e.bytes} for e in arr[:n.value]}
+
     def last_factor_ms(self):
         v = ctypes.c_double()
         _lib.check(lib().cliqg_plan_last_factor_ms(self._h, ctypes.byref(v)))

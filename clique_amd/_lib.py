@@ -20,6 +20,12 @@ class CliqgError(RuntimeError):
         self.code = code
 
 
+class ProfileEntry(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_int),
+                ("total_ms", ctypes.c_double), ("flops", ctypes.c_double),
+                ("bytes", ctypes.c_double)]
+
+
 class RefineInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
                 ("max_backward_error", ctypes.c_double)]
@@ -41,7 +47,7 @@ EXPORTS = [
     "cliqg_symbolic_work", "cliqg_plan_create", "cliqg_plan_destroy", "cliqg_plan_memory",
     "cliqg_plan_factor", "cliqg_plan_solve", "cliqg_plan_solve_nodal", "cliqg_plan_multiply",
     "cliqg_plan_refine", "cliqg_plan_adjoint", "cliqg_plan_get_front",
-    "cliqg_plan_last_factor_ms",
+    "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
 ]
 
 
@@ -86,6 +92,8 @@ def lib():
         "cliqg_plan_adjoint": (c_int, [vp, vp, c_int, vp, c_int, c_int, vp, c_int, vp]),
         "cliqg_plan_get_front": (c_int, [vp, c_int, vp, vp]),
         "cliqg_plan_last_factor_ms": (c_int, [vp, dp]),
+        "cliqg_plan_set_profiling": (c_int, [vp, c_int]),
+        "cliqg_plan_get_profile": (c_int, [vp, ctypes.POINTER(ProfileEntry), c_int, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

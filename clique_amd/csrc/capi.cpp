@@ -308,4 +308,35 @@ cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms) {
     });
 }
 
+cliqg_status cliqg_plan_set_profiling(cliqg_plan plan, int enable) {
+    return guarded([&] {
+        auto& p = P(plan);
+        p.profiling = enable != 0;
+        p.profile.clear();
+        p.pending.clear();
+    });
+}
+
+cliqg_status cliqg_plan_get_profile(cliqg_plan plan, cliqg_profile_entry* h_out, int capacity,
+                                    int* count) {
+    return guarded([&] {
+        auto& p = P(plan);
+        require(count, "null argument");
+        int n = 0;
+        for (const auto& kv : p.profile) {
+            if (h_out && n < capacity) {
+                cliqg_profile_entry& e = h_out[n];
+                std::memset(e.name, 0, sizeof(e.name));
+                std::strncpy(e.name, kv.first.c_str(), sizeof(e.name) - 1);
+                e.launches = kv.second.launches;
+                e.total_ms = kv.second.ms;
+                e.flops = kv.second.flops;
+                e.bytes = kv.second.bytes;
+            }
+            ++n;
+        }
+        *count = n;
+    });
+}
+
 }  // extern "C"

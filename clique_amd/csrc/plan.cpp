@@ -114,17 +114,35 @@ struct Step {
     size_t desc, start;   // arena offsets
     int ndesc, nwork;
     bool backward;
+    double flops = 0.0;   // algorithmic flops of the launch (reference convention)
+    double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
 };
+
+const char* step_name(int kind) {
+    switch (kind) {
+        case 0: return "extend_add";
+        case 1: return "diag_ldl";
+        case 2: return "gemm";
+        case 3: return "solve_copy_in";
+        case 4: return "solve_copy_out";
+        case 5: return "solve_extend_add";
+        case 6: return "zero_update";
+        default: return "other";
+    }
+}
 
 template <class T>
 struct StepBuilder {
     Arena& arena;
     std::vector<Step>& steps;
 
-    void gemm(const std::vector<GemmDesc<T>>& in) {
+    // triB: op(B) is a triangular K x K block (the inverted diagonal blocks),
+    // so only half of the K products count as algorithmic work.
+    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) {
         std::vector<GemmDesc<T>> ds;
         std::vector<int> start;
         int total = 0;
+        double flops = 0.0, bytes = 0.0;
         for (const auto& d : in) {
             const int t = gemm_tiles(d.M, d.N, d.flags);
             if (t == 0) continue;
@@ -132,9 +150,14 @@ struct StepBuilder {
             ds.push_back(d);
             start.push_back(total);
             total += t;
+            const double M = d.M, N = d.N, K = d.K;
+            const double entries = (d.flags & G_LOWER) ? M * N - N * (N - 1) / 2 : M * N;
+            const double keff = triB ? (K + 1) / 2 : K;
+            flops += 2.0 * entries * keff * (Scalar<T>::complex ? 4 : 1);
+            bytes += (M * K + K * N + entries * ((d.flags & G_ACCUM) ? 2 : 1)) * sizeof(T);
         }
         if (ds.empty()) return;
-        Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false};
+        Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false, flops, bytes};
         steps.push_back(s);
     }
 };
@@ -218,9 +241,12 @@ struct Numeric : NumericBase {
                     start.push_back(total);
                     total += fc.u;
                 }
-                if (!ds.empty())
+                if (!ds.empty()) {
+                    double by = 0.0;
+                    for (const auto& x : ds) by += 3.0 * x.uc * (x.uc + 1.0) / 2.0 * sizeof(T);
                     fsteps.push_back(Step{S_EXTEND, farena.add(ds), farena.add(start),
-                                          (int)ds.size(), total, false});
+                                          (int)ds.size(), total, false, 0.0, by});
+                }
             }
             int maxn = 0;
             for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
@@ -247,9 +273,13 @@ struct Numeric : NumericBase {
                                                   G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
                     }
                 }
-                if (!dd.empty())
-                    fsteps.push_back(Step{S_DIAG, farena.add(dd), 0, (int)dd.size(), 0, false});
-                sb.gemm(trsm);
+                if (!dd.empty()) {
+                    double fl = 0.0;
+                    for (const auto& x : dd) fl += (double)x.b * x.b * x.b / 3.0;
+                    fsteps.push_back(Step{S_DIAG, farena.add(dd), 0, (int)dd.size(), 0, false,
+                                          fl * (Scalar<T>::complex ? 4 : 1), 0.0});
+                }
+                sb.gemm(trsm, true);
                 sb.gemm(upd);
             }
             std::vector<GemmDesc<T>> syrk;
@@ -327,7 +357,7 @@ struct Numeric : NumericBase {
                                                          w + k0 + b, nullptr, nullptr, f.m, f.m, f.m,
                                                          f.m - k0 - b, k, b, G_ACCUM | G_NEG});
                         }
-                        sb.gemm(tr);
+                        sb.gemm(tr, true);
                         sb.gemm(up);
                     }
                 } else {
@@ -356,7 +386,7 @@ struct Numeric : NumericBase {
                                                          f.m, f.m, k0, k, b,
                                                          G_TRANSA | G_ACCUM | G_NEG | cA});
                         }
-                        sb.gemm(tr);
+                        sb.gemm(tr, true);
                         sb.gemm(up);
                     }
                 }
@@ -372,6 +402,11 @@ struct Numeric : NumericBase {
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
         hipStream_t st = P.stream;
         for (const Step& s : steps) {
+            hipEvent_t e0 = nullptr;
+            if (P.profiling) {
+                e0 = P.next_event();
+                CQ_HIP(hipEventRecord(e0, st));
+            }
             switch (s.kind) {
                 case S_ZERO:
                     launch_zero_segments<T>(a.at<ZeroSeg<T>>(s.desc), a.at<int>(s.start), s.ndesc,
@@ -402,6 +437,11 @@ struct Numeric : NumericBase {
                     launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
                                            s.nwork, S->wpool.template as<T>(), P.dRel.as<int>(), k, st);
                     break;
+            }
+            if (P.profiling) {
+                hipEvent_t e1 = P.next_event();
+                CQ_HIP(hipEventRecord(e1, st));
+                P.pending.push_back(Plan::Pending{step_name(s.kind), s.flops, s.bytes, e0, e1});
             }
         }
         CQ_HIP(hipGetLastError());
@@ -581,6 +621,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
 
 Plan::~Plan() {
     num.reset();
+    for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (ownStream && stream) (void)hipStreamDestroy(stream);
@@ -593,10 +634,34 @@ size_t Plan::device_bytes() const {
     return b;
 }
 
+hipEvent_t Plan::next_event() {
+    if (evNext_ == evPool_.size()) {
+        hipEvent_t e;
+        CQ_HIP(hipEventCreate(&e));
+        evPool_.push_back(e);
+    }
+    return evPool_[evNext_++];
+}
+
+void Plan::resolve_profile() {
+    for (const Pending& p : pending) {
+        float ms = 0.f;
+        CQ_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+        ProfileEntry& e = profile[p.name];
+        e.launches += 1;
+        e.ms += ms;
+        e.flops += p.flops;
+        e.bytes += p.bytes;
+    }
+    pending.clear();
+    evNext_ = 0;
+}
+
 int Plan::check_status(const char* what) {
     int st[3] = {0, 0, 0};
     CQ_HIP(hipMemcpyAsync(st, dStatus.p, sizeof(st), hipMemcpyDeviceToHost, stream));
     CQ_HIP(hipStreamSynchronize(stream));
+    if (profiling) resolve_profile();
     if (st[0] != 0)
         throw Error(2, std::string(what) + ": zero or non-finite pivot in front " +
                            std::to_string(st[1]) + " at column " + std::to_string(st[2]));

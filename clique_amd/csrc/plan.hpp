@@ -1,6 +1,7 @@
 // Device plan: front layout in HBM, static level schedule, numeric drivers.
 #pragma once
 #include <map>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -85,9 +86,27 @@ public:
     std::unique_ptr<NumericBase> num;
     int check_status(const char* what);
 
+    // opt-in per-launch profiling (HIP events around every scheduled launch)
+    struct Pending {
+        const char* name;
+        double flops, bytes;
+        hipEvent_t e0, e1;
+    };
+    struct ProfileEntry {
+        int launches = 0;
+        double ms = 0.0, flops = 0.0, bytes = 0.0;
+    };
+    bool profiling = false;
+    std::vector<Pending> pending;
+    std::map<std::string, ProfileEntry> profile;
+    hipEvent_t next_event();
+    void resolve_profile();
+
 private:
     double lastFactorMs_ = 0.0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<hipEvent_t> evPool_;
+    size_t evNext_ = 0;
 };
 
 }  // namespace cliqg

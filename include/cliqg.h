@@ -174,6 +174,22 @@ cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_dia
 /* Timing of the last factorization on the plan's stream (HIP events). */
 cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms);
 
+/* Opt-in profiling (the role of the reference's per-phase timers and
+ * FactorizationWork accounting, tests/Solve.cpp:223-251): HIP events around
+ * every scheduled launch, accumulated per kernel family with the launch's
+ * algorithmic flops (FactorizationWork convention) and bytes. Enabling
+ * resets the accumulators. */
+typedef struct {
+    char name[32];
+    int launches;
+    double total_ms;
+    double flops;
+    double bytes;
+} cliqg_profile_entry;
+cliqg_status cliqg_plan_set_profiling(cliqg_plan plan, int enable);
+cliqg_status cliqg_plan_get_profile(cliqg_plan plan, cliqg_profile_entry* h_out, int capacity,
+                                    int* count);
+
 #ifdef __cplusplus
 }
 #endif
