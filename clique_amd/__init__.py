@@ -138,9 +138,11 @@ class Symbolic:
 
 def NaturalNestedDissection(nx, ny, nz, indptr, indices, cutoff=128):
     """Natural nested dissection of an nx*ny*nz grid graph given as CSR."""
-    N = int(nx) * int(ny) * int(nz)
     ip = np.ascontiguousarray(indptr, dtype=np.int64)
     ind = np.ascontiguousarray(indices, dtype=np.int32)
+    N = ip.size - 1
+    if N < 0 or ind.size < ip[-1]:
+        raise ValueError("malformed CSR arrays")
     h = ctypes.c_void_p()
     _lib.check(lib().cliqg_symbolic_natural(
         int(nx), int(ny), int(nz), N, ip.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),

@@ -73,7 +73,8 @@ int gemm_tiles(int M, int N, int flags) {
 // owns a 32x32 quadrant as 2x2 MFMA 16x16 blocks. Complex products use four
 // real MFMAs on planar (re, im) LDS tiles.
 // ---------------------------------------------------------------------------
-template <class T>
+// ROLE only separates the kernel's name in traces (0: factorization, 1: solves).
+template <class T, int ROLE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ descs,
                                               const int* __restrict__ tile_start, int ndesc) {
     using R = typename Scalar<T>::real;
@@ -233,9 +234,14 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
 
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
-                 hipStream_t s) {
+                 int role, hipStream_t s) {
     if (ntiles <= 0 || ndesc <= 0) return;
-    hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start, ndesc);
+    if (role == 0)
+        hipLaunchKernelGGL((k_gemm<T, 0>), dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start,
+                           ndesc);
+    else
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start,
+                           ndesc);
 }
 
 // ---------------------------------------------------------------------------
@@ -675,7 +681,7 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 #define CQ_INST_T(T)                                                                           \
     template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
-    template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, hipStream_t);        \
+    template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \
     template void launch_copy_in<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,   \

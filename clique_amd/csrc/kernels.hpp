@@ -90,7 +90,7 @@ void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, in
                           int nchunks, hipStream_t s);
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
-                 hipStream_t s);
+                 int role, hipStream_t s);
 // number of output tiles of a descriptor (host side, must match the kernel)
 int gemm_tiles(int M, int N, int flags);
 

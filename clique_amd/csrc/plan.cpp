@@ -123,6 +123,7 @@ const char* step_name(int kind) {
         case 0: return "extend_add";
         case 1: return "diag_ldl";
         case 2: return "gemm";
+        case 7: return "solve_gemm";
         case 3: return "solve_copy_in";
         case 4: return "solve_copy_out";
         case 5: return "solve_extend_add";
@@ -422,7 +423,7 @@ struct Numeric : NumericBase {
                     break;
                 case S_GEMM:
                     launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                   st);
+                                   S ? 1 : 0, st);
                     break;
                 case S_COPYIN:
                     launch_copy_in<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
@@ -441,7 +442,8 @@ struct Numeric : NumericBase {
             if (P.profiling) {
                 hipEvent_t e1 = P.next_event();
                 CQ_HIP(hipEventRecord(e1, st));
-                P.pending.push_back(Plan::Pending{step_name(s.kind), s.flops, s.bytes, e0, e1});
+                const int kind = (s.kind == S_GEMM && S) ? 7 : s.kind;
+                P.pending.push_back(Plan::Pending{step_name(kind), s.flops, s.bytes, e0, e1});
             }
         }
         CQ_HIP(hipGetLastError());
