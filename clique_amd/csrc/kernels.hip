@@ -58,9 +58,15 @@ __host__ __device__ __forceinline__ long long lower_prefix(long long tj, long lo
 
 }  // namespace
 
-int gemm_tiles(int M, int N, int flags) {
+int gemm_tile_m(int variant) { return variant == 0 ? 64 : 128; }
+int gemm_tile_n(int variant) { return variant == 2 ? 128 : 64; }
+
+int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
-    const int tm = (M + GEMM_TM - 1) / GEMM_TM, tn = (N + GEMM_TN - 1) / GEMM_TN;
+    const int TM = gemm_tile_m(variant), TN = gemm_tile_n(variant);
+    const int tm = (M + TM - 1) / TM, tn = (N + TN - 1) / TN;
+    if ((flags & G_LOWER) && TM != TN)
+        throw Error(1, "lower-masked GEMM launches need square tiles");
     if (flags & G_LOWER) {
         const int c = tn < tm ? tn : tm;
         return (int)lower_prefix(c, tm);
@@ -69,12 +75,13 @@ int gemm_tiles(int M, int N, int flags) {
 }
 
 // ---------------------------------------------------------------------------
-// Grouped GEMM: one 64x64 output tile per workgroup of 4 waves; each wave
-// owns a 32x32 quadrant as 2x2 MFMA 16x16 blocks. Complex products use four
-// real MFMAs on planar (re, im) LDS tiles.
+// Grouped GEMM: one TM x TN output tile per workgroup of 4 waves (2 x 2);
+// each wave owns a (TM/2) x (TN/2) quadrant of 16x16 MFMA blocks. K is
+// consumed in chunks of 16 through two LDS buffers (one barrier per chunk);
+// the next chunk is prefetched into registers while the current one feeds
+// the MFMAs. Complex products use four real MFMAs on planar (re, im) tiles.
 // ---------------------------------------------------------------------------
-// ROLE only separates the kernel's name in traces (0: factorization, 1: solves).
-template <class T, int ROLE>
+template <class T, int TM, int TN, int ROLE>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ descs,
                                               const int* __restrict__ tile_start, int ndesc) {
     using R = typename Scalar<T>::real;
@@ -82,15 +89,18 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
     using A4 = typename Acc4<R>::type;
     constexpr bool CPX = Scalar<T>::complex;
     constexpr int NP = CPX ? 2 : 1;
+    constexpr int BM = TM / 32, BN = TN / 32;      // 16x16 blocks per wave
+    constexpr int LA = TM * TK / 256, LB = TN * TK / 256;
+    constexpr bool F64 = sizeof(R) == 8;
 
-    __shared__ R As[NP][TK][GEMM_TM + LPAD];
-    __shared__ R Bs[NP][TK][GEMM_TN + LPAD];
+    __shared__ R As[2][NP][TK][TM + LPAD];
+    __shared__ R Bs[2][NP][TK][TN + LPAD];
 
     const int tile = blockIdx.x;
     const int di = find_segment(tile_start, ndesc, tile);
     const GemmDesc<T> d = descs[di];
     const int t = tile - tile_start[di];
-    const int tilesM = (d.M + GEMM_TM - 1) / GEMM_TM;
+    const int tilesM = (d.M + TM - 1) / TM;
     int ti, tj;
     if (d.flags & G_LOWER) {
         const double b2 = 2.0 * tilesM + 1.0;
@@ -104,7 +114,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
         ti = t % tilesM;
         tj = t / tilesM;
     }
-    const int row0 = ti * GEMM_TM, col0 = tj * GEMM_TN;
+    const int row0 = ti * TM, col0 = tj * TN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave & 1, wj = wave >> 1;
@@ -112,22 +122,22 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
     const bool conjA = d.flags & G_CONJA, conjB = d.flags & G_CONJB;
     const W* scaleK = reinterpret_cast<const W*>(d.scaleK);
 
-    A4 accR[2][2], accI[2][2];
+    A4 accR[BM][BN], accI[BM][BN];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < BN; ++b) {
             accR[a][b] = A4{0, 0, 0, 0};
             accI[a][b] = A4{0, 0, 0, 0};
         }
 
-    T ra[4], rb[4];
+    T ra[LA], rb[LB];
     auto load = [&](int k0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < LA; ++r) {
             const int e = tid + 256 * r;
             int i, k;
-            if (!transA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
+            if (!transA) { i = e % TM; k = e / TM; } else { k = e & 15; i = e >> 4; }
             const int gi = row0 + i, gk = k0 + k;
             T v = zero<T>();
             if (gi < d.M && gk < d.K) {
@@ -138,10 +148,10 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
             ra[r] = v;
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < LB; ++r) {
             const int e = tid + 256 * r;
             int j, k;
-            if (transB) { j = e & 63; k = e >> 6; } else { k = e & 15; j = e >> 4; }
+            if (transB) { j = e % TN; k = e / TN; } else { k = e & 15; j = e >> 4; }
             const int gj = col0 + j, gk = k0 + k;
             T v = zero<T>();
             if (gj < d.N && gk < d.K) {
@@ -151,50 +161,52 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
             rb[r] = v;
         }
     };
-    auto store = [&]() {
+    auto store = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < LA; ++r) {
             const int e = tid + 256 * r;
             int i, k;
-            if (!transA) { i = e & 63; k = e >> 6; } else { k = e & 15; i = e >> 4; }
-            As[0][k][i] = re(ra[r]);
-            if constexpr (CPX) As[NP - 1][k][i] = im(ra[r]);
+            if (!transA) { i = e % TM; k = e / TM; } else { k = e & 15; i = e >> 4; }
+            As[buf][0][k][i] = re(ra[r]);
+            if constexpr (CPX) As[buf][NP - 1][k][i] = im(ra[r]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < LB; ++r) {
             const int e = tid + 256 * r;
             int j, k;
-            if (transB) { j = e & 63; k = e >> 6; } else { k = e & 15; j = e >> 4; }
-            Bs[0][k][j] = re(rb[r]);
-            if constexpr (CPX) Bs[NP - 1][k][j] = im(rb[r]);
+            if (transB) { j = e % TN; k = e / TN; } else { k = e & 15; j = e >> 4; }
+            Bs[buf][0][k][j] = re(rb[r]);
+            if constexpr (CPX) Bs[buf][NP - 1][k][j] = im(rb[r]);
         }
     };
 
     const int nk = (d.K + TK - 1) / TK;
     if (nk > 0) {
         load(0);
-        store();
+        store(0);
         __syncthreads();
     }
     for (int c = 0; c < nk; ++c) {
+        const int buf = c & 1;
         if (c + 1 < nk) load((c + 1) * TK);
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
-            R aR[2], aI[2], bR[2], bI[2];
+            R aR[BM], aI[BM], bR[BN], bI[BN];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                aR[q] = As[0][kr][wi * 32 + q * 16 + (lane & 15)];
-                bR[q] = Bs[0][kr][wj * 32 + q * 16 + (lane & 15)];
-                if constexpr (CPX) {
-                    aI[q] = As[NP - 1][kr][wi * 32 + q * 16 + (lane & 15)];
-                    bI[q] = Bs[NP - 1][kr][wj * 32 + q * 16 + (lane & 15)];
-                }
+            for (int q = 0; q < BM; ++q) {
+                aR[q] = As[buf][0][kr][wi * (TM / 2) + q * 16 + (lane & 15)];
+                if constexpr (CPX) aI[q] = As[buf][NP - 1][kr][wi * (TM / 2) + q * 16 + (lane & 15)];
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int q = 0; q < BN; ++q) {
+                bR[q] = Bs[buf][0][kr][wj * (TN / 2) + q * 16 + (lane & 15)];
+                if constexpr (CPX) bI[q] = Bs[buf][NP - 1][kr][wj * (TN / 2) + q * 16 + (lane & 15)];
+            }
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+            for (int a = 0; a < BM; ++a)
+#pragma unroll
+                for (int b = 0; b < BN; ++b) {
                     accR[a][b] = mfma16(aR[a], bR[b], accR[a][b]);
                     if constexpr (CPX) {
                         accR[a][b] = mfma16(-aI[a], bI[b], accR[a][b]);
@@ -203,25 +215,21 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
                     }
                 }
         }
+        if (c + 1 < nk) store(buf ^ 1);
         __syncthreads();
-        if (c + 1 < nk) {
-            store();
-            __syncthreads();
-        }
     }
 
     const W* scaleN = reinterpret_cast<const W*>(d.scaleN);
     const bool lower = d.flags & G_LOWER, accum = d.flags & G_ACCUM, neg = d.flags & G_NEG;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < BM; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < BN; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                constexpr bool F64 = sizeof(R) == 8;
                 const int rin = F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                const int gi = row0 + wi * 32 + a * 16 + rin;
-                const int gj = col0 + wj * 32 + b * 16 + (lane & 15);
+                const int gi = row0 + wi * (TM / 2) + a * 16 + rin;
+                const int gj = col0 + wj * (TN / 2) + b * 16 + (lane & 15);
                 if (gi >= d.M || gj >= d.N || (lower && gi < gj)) continue;
                 T v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
                 if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
@@ -232,16 +240,29 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
             }
 }
 
+template <class T, int TM, int TN>
+static void launch_gemm_shape(const GemmDesc<T>* d, const int* ts, int nd, int nt, int role,
+                              hipStream_t s) {
+    if (role == 0)
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), dim3(nt), dim3(256), 0, s, d, ts, nd);
+    else
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), dim3(nt), dim3(256), 0, s, d, ts, nd);
+}
+
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
-                 int role, hipStream_t s) {
+                 int variant, int role, hipStream_t s) {
     if (ntiles <= 0 || ndesc <= 0) return;
-    if (role == 0)
-        hipLaunchKernelGGL((k_gemm<T, 0>), dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start,
-                           ndesc);
-    else
-        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(ntiles), dim3(256), 0, s, d_descs, d_tile_start,
-                           ndesc);
+    if (variant == 2) {
+        if constexpr (gemm_variant_allowed<T>(2))
+            launch_gemm_shape<T, 128, 128>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+        else
+            throw Error(1, "128x128 GEMM tiles are not built for complex double");
+    } else if (variant == 1) {
+        launch_gemm_shape<T, 128, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+    } else {
+        launch_gemm_shape<T, 64, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -324,7 +345,14 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 
 // ---------------------------------------------------------------------------
 // Diagonal block: unpivoted LDL^T of a b x b block (b <= NB) in FP64 (field
-// type) in LDS, unit lower inverse of its L. One workgroup per front.
+// type) in LDS, and the inverse of its unit lower factor. One workgroup per
+// front, one barrier per column: step j reads the (unscaled) column j and
+// applies both the rank-1 Schur update of the trailing block,
+//     a(i,c) -= a(i,j) op(a(c,j)) / d_j          (j < c <= i),
+// and the matching row operation on the inverse,
+//     li(i,c) -= (a(i,j) / d_j) li(j,c)          (c <= j < i),
+// neither of which touches column j or row j. Column j is scaled by 1/d_j
+// once all steps are done.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
@@ -333,6 +361,7 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
     extern __shared__ unsigned char smem_raw[];
     W (*a)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw);
     W (*li)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw + sizeof(W) * NB * (NB + 1));
+    W* dinv = reinterpret_cast<W*>(smem_raw + 2 * sizeof(W) * NB * (NB + 1));
 
     const DiagDesc<T> d = descs[blockIdx.x];
     const int b = d.b, tid = threadIdx.x;
@@ -341,41 +370,43 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
         W v = W{};
         if (i < b && j < b && i >= j) v = cvt<W>(d.A11[i + (int64_t)j * d.lda]);
         a[i][j] = v;
-        li[i][j] = W{};
+        li[i][j] = (i == j) ? one<W>() : W{};
     }
     W* Dout = reinterpret_cast<W*>(d.D);
     for (int j = 0; j < b; ++j) {
         __syncthreads();
         W piv = a[j][j];
         if (herm) piv = real_part_only(piv);
+        const W ip = one<W>() / piv;
         if (tid == 0) {
             if (is_zero_or_nonfinite(piv)) record_status(status, 2, d.front, d.col0 + j);
             Dout[j] = piv;
+            dinv[j] = ip;
         }
-        for (int i = j + 1 + tid; i < b; i += blockDim.x) a[i][j] = a[i][j] / piv;
-        __syncthreads();
-        const int r = b - j - 1;
-        for (int idx = tid; idx < r * r; idx += blockDim.x) {
-            const int ii = j + 1 + idx % r, cc = j + 1 + idx / r;
-            if (cc <= ii) a[ii][cc] = a[ii][cc] - a[ii][j] * piv * cj(a[cc][j], herm);
-        }
-    }
-    __syncthreads();
-    // unit lower inverse, rows in sequence, columns in parallel
-    if (tid < b) li[tid][tid] = one<W>();
-    for (int i = 1; i < b; ++i) {
-        __syncthreads();
-        if (tid < i) {
-            const int j = tid;
-            W s = W{};
-            for (int l = j; l < i; ++l) s = s + a[i][l] * li[l][j];
-            li[i][j] = -s;
+        const int r = b - j - 1;          // rows below the pivot
+        const int nt = r * (r + 1) / 2;   // trailing lower entries
+        const int nl = r * (j + 1);       // inverse entries updated
+        for (int idx = tid; idx < nt + nl; idx += blockDim.x) {
+            if (idx < nt) {
+                // (ii, cc) with cc <= ii over the r x r trailing triangle
+                int cc = (int)((sqrtf(8.f * idx + 1.f) - 1.f) * 0.5f);
+                while (cc * (cc + 1) / 2 > idx) --cc;
+                while ((cc + 1) * (cc + 2) / 2 <= idx) ++cc;
+                const int ii = idx - cc * (cc + 1) / 2;   // 0..cc
+                // map (row-major packed) to ii >= cc ordering: row = j+1+cc, col = j+1+ii
+                const int row = j + 1 + cc, col = j + 1 + ii;
+                a[row][col] = a[row][col] - a[row][j] * ip * cj(a[col][j], herm);
+            } else {
+                const int q = idx - nt;
+                const int row = j + 1 + q / (j + 1), col = q % (j + 1);
+                li[row][col] = li[row][col] - a[row][j] * ip * li[j][col];
+            }
         }
     }
     __syncthreads();
     for (int idx = tid; idx < b * b; idx += blockDim.x) {
         const int i = idx % b, j = idx / b;
-        if (i > j) d.A11[i + (int64_t)j * d.lda] = cvt<T>(a[i][j]);
+        if (i > j) d.A11[i + (int64_t)j * d.lda] = cvt<T>(a[i][j] * dinv[j]);
         else if (i == j) d.A11[i + (int64_t)j * d.lda] = one<T>();
     }
     for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
@@ -389,7 +420,7 @@ void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_st
                      hipStream_t s) {
     if (ndesc <= 0) return;
     using W = typename Scalar<T>::wide;
-    const size_t shm = 2 * sizeof(W) * NB * (NB + 1);
+    const size_t shm = 2 * sizeof(W) * NB * (NB + 1) + sizeof(W) * NB;
     static bool attr_set = false;
     if (!attr_set) {
         CQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag_ldl<T>),
@@ -535,18 +566,32 @@ void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream
 // ---------------------------------------------------------------------------
 // FP64 sparse kernels
 // ---------------------------------------------------------------------------
+// y := alpha A x + beta y over CSR rows; one thread per (row, 8 right-hand
+// sides): the row's indices and values are read once per chunk of columns.
+constexpr int SPMV_COLS = 8;
 template <class V>
 __global__ void k_spmv(int N, const int64_t* __restrict__ rowoff, const int* __restrict__ cols,
                        const V* __restrict__ vals, double alpha, const V* __restrict__ x,
                        int64_t ldx, double beta, V* __restrict__ y, int64_t ldy, int k) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(g % N), j = (int)(g / N);
-        V acc = V{};
-        for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) acc = acc + vals[e] * x[cols[e] + j * ldx];
-        V* py = y + i + j * ldy;
-        V out = scale(acc, alpha);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int j0 = blockIdx.y * SPMV_COLS;
+    const int nj = min(SPMV_COLS, k - j0);
+    V acc[SPMV_COLS];
+#pragma unroll
+    for (int q = 0; q < SPMV_COLS; ++q) acc[q] = V{};
+    for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) {
+        const V a = vals[e];
+        const V* xc = x + cols[e] + j0 * ldx;
+#pragma unroll
+        for (int q = 0; q < SPMV_COLS; ++q)
+            if (q < nj) acc[q] = acc[q] + a * xc[q * ldx];
+    }
+#pragma unroll
+    for (int q = 0; q < SPMV_COLS; ++q) {
+        if (q >= nj) break;
+        V* py = y + i + (j0 + q) * ldy;
+        V out = scale(acc[q], alpha);
         if (beta != 0.0) out = out + scale(*py, beta);
         *py = out;
     }
@@ -554,8 +599,9 @@ __global__ void k_spmv(int N, const int64_t* __restrict__ rowoff, const int* __r
 template <class V>
 void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
                  const V* x, int64_t ldx, double beta, V* y, int64_t ldy, int k, hipStream_t s) {
-    hipLaunchKernelGGL(k_spmv<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, N, rowoff, cols,
-                       vals, alpha, x, ldx, beta, y, ldy, k);
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_spmv<V>, dim3((N + 255) / 256, (k + SPMV_COLS - 1) / SPMV_COLS), dim3(256),
+                       0, s, N, rowoff, cols, vals, alpha, x, ldx, beta, y, ldy, k);
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
@@ -681,7 +727,7 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 #define CQ_INST_T(T)                                                                           \
     template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
-    template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, hipStream_t);        \
+    template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \
     template void launch_copy_in<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,   \

@@ -10,9 +10,14 @@ namespace cliqg {
 
 // Width of a diagonal block (panel) of the partial LDL^T.
 constexpr int NB = 64;
-// Tile of the grouped GEMM.
-constexpr int GEMM_TM = 64;
-constexpr int GEMM_TN = 64;
+// Tile shapes of the grouped GEMM: variant 0 = 64x64, 1 = 128x64,
+// 2 = 128x128 (not for complex double: its accumulators would not fit).
+int gemm_tile_m(int variant);
+int gemm_tile_n(int variant);
+template <class T>
+constexpr bool gemm_variant_allowed(int variant) {
+    return variant < 2 || !(Scalar<T>::complex && sizeof(T) == 16);
+}
 
 enum GemmFlags : int {
     G_TRANSA = 1,   // op(A)(i,k) = A[k + i*lda]   (else A[i + k*lda])
@@ -90,9 +95,9 @@ void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, in
                           int nchunks, hipStream_t s);
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
-                 int role, hipStream_t s);
+                 int variant, int role, hipStream_t s);
 // number of output tiles of a descriptor (host side, must match the kernel)
-int gemm_tiles(int M, int N, int flags);
+int gemm_tiles(int M, int N, int flags, int variant);
 
 template <class T, class V>
 void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bool conj,

@@ -116,6 +116,7 @@ struct Step {
     bool backward;
     double flops = 0.0;   // algorithmic flops of the launch (reference convention)
     double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
+    int variant = 0;      // GEMM tile shape
 };
 
 const char* step_name(int kind) {
@@ -144,8 +145,9 @@ struct StepBuilder {
         std::vector<int> start;
         int total = 0;
         double flops = 0.0, bytes = 0.0;
+        const int variant = choose_variant(in);
         for (const auto& d : in) {
-            const int t = gemm_tiles(d.M, d.N, d.flags);
+            const int t = gemm_tiles(d.M, d.N, d.flags, variant);
             if (t == 0) continue;
             if (d.K == 0 && (d.flags & G_ACCUM)) continue;
             ds.push_back(d);
@@ -159,7 +161,35 @@ struct StepBuilder {
         }
         if (ds.empty()) return;
         Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false, flops, bytes};
+        s.variant = variant;
         steps.push_back(s);
+    }
+
+    // Tile shape of a grouped launch: padded MFMA work divided by the share
+    // of the 256 CUs the launch can occupy, weighted by the measured relative
+    // cost per flop of each tile shape on gfx950 (the 128-row tiles run at one
+    // or two waves per SIMD and lose to 64x64 on every measured shape).
+    static int choose_variant(const std::vector<GemmDesc<T>>& in) {
+        static const double rel[3] = {1.0, 1.35, 1.6};
+        int best = 0;
+        double bestCost = 1e300;
+        bool lower = false;
+        for (const auto& d : in) lower |= (d.flags & G_LOWER) != 0;
+        for (int v = 0; v < 3; ++v) {
+            if (!gemm_variant_allowed<T>(v)) continue;
+            if (lower && gemm_tile_m(v) != gemm_tile_n(v)) continue;
+            double padded = 0.0, tiles = 0.0;
+            for (const auto& d : in) {
+                const int t = gemm_tiles(d.M, d.N, d.flags, v);
+                tiles += t;
+                padded += (double)t * gemm_tile_m(v) * gemm_tile_n(v) * std::max(d.K, 1);
+            }
+            if (tiles == 0) continue;
+            const double occ = std::min(1.0, tiles / 256.0);
+            const double cost = rel[v] * padded / occ;
+            if (cost < bestCost) { bestCost = cost; best = v; }
+        }
+        return best;
     }
 };
 
@@ -423,7 +453,7 @@ struct Numeric : NumericBase {
                     break;
                 case S_GEMM:
                     launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                   S ? 1 : 0, st);
+                                   s.variant, S ? 1 : 0, st);
                     break;
                 case S_COPYIN:
                     launch_copy_in<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
