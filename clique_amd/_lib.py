@@ -46,6 +46,7 @@ EXPORTS = [
     "cliqg_symbolic_node", "cliqg_symbolic_node_array", "cliqg_symbolic_map",
     "cliqg_symbolic_work", "cliqg_plan_create", "cliqg_plan_destroy", "cliqg_plan_memory",
     "cliqg_plan_factor", "cliqg_plan_solve", "cliqg_plan_solve_nodal", "cliqg_plan_multiply",
+    "cliqg_spmv",
     "cliqg_plan_refine", "cliqg_plan_adjoint", "cliqg_plan_get_front",
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
 ]
@@ -87,6 +88,8 @@ def lib():
         "cliqg_plan_solve": (c_int, [vp, vp, c_int, c_int]),
         "cliqg_plan_solve_nodal": (c_int, [vp, vp, c_int, c_int]),
         "cliqg_plan_multiply": (c_int, [vp, c_double, vp, c_int, c_double, vp, c_int, c_int]),
+        "cliqg_spmv": (c_int, [c_int, vp, vp, vp, c_int, c_double, vp, c_int, c_double, vp, c_int,
+                                c_int, vp]),
         "cliqg_plan_refine": (c_int, [vp, vp, c_int, vp, c_int, c_int, c_double, c_int,
                                       ctypes.POINTER(RefineInfo)]),
         "cliqg_plan_adjoint": (c_int, [vp, vp, c_int, vp, c_int, c_int, vp, c_int, vp]),

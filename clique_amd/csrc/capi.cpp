@@ -268,6 +268,25 @@ cliqg_status cliqg_plan_multiply(cliqg_plan plan, double alpha, const void* d_x,
     });
 }
 
+cliqg_status cliqg_spmv(int N, const int64_t* d_row_off, const int* d_cols, const void* d_vals,
+                        cliqg_field field, double alpha, const void* d_x, int ldx, double beta,
+                        void* d_y, int ldy, int k, void* hip_stream) {
+    return guarded([&] {
+        require(N >= 0 && k >= 0 && ldx >= N && ldy >= N, "bad dimensions");
+        if (N == 0 || k == 0) return;
+        require(d_row_off && d_cols && d_vals && d_x && d_y, "null argument");
+        hipStream_t st = static_cast<hipStream_t>(hip_stream);
+        if (field == CLIQG_COMPLEX)
+            cliqg::launch_spmv<cliqg::cdouble>(N, d_row_off, d_cols, (const cliqg::cdouble*)d_vals,
+                                               alpha, (const cliqg::cdouble*)d_x, ldx, beta,
+                                               (cliqg::cdouble*)d_y, ldy, k, st);
+        else
+            cliqg::launch_spmv<double>(N, d_row_off, d_cols, (const double*)d_vals, alpha,
+                                       (const double*)d_x, ldx, beta, (double*)d_y, ldy, k, st);
+        CQ_HIP(hipGetLastError());
+    });
+}
+
 cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* d_x, int ldx,
                                int k, double tol, int max_iters, cliqg_refine_info* info) {
     return guarded([&] {

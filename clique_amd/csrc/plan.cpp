@@ -834,13 +834,23 @@ void Plan::adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, i
                    void* d_lambda, int64_t ldl, void* d_dA) {
     if (!factored) throw Error(6, "the plan has not been factored");
     if (k <= 0) return;
+    if (precision == 1) {
+        // FP32 factor: lambda is refined in FP64 like the forward solution
+        int it = 0, conv = 0;
+        double eta = 0.0;
+        refine(d_g, ldg, d_lambda, ldl, k, 1e-12, 30, &it, &conv, &eta);
+        if (!conv)
+            throw Error(3, "adjoint refinement stopped at backward error " + std::to_string(eta));
+    }
     dispatch(complex, precision, [&](auto* tag) {
         using T = std::remove_pointer_t<decltype(tag)>;
         using V = typename Scalar<T>::wide;
         auto* nm = static_cast<Numeric<T>*>(num.get());
         V* lam = static_cast<V*>(d_lambda);
-        launch_copy_cols<V>(lam, ldl, static_cast<const V*>(d_g), ldg, N, k, stream);
-        nm->solve(lam, ldl, k, false);
+        if (precision == 0) {
+            launch_copy_cols<V>(lam, ldl, static_cast<const V*>(d_g), ldg, N, k, stream);
+            nm->solve(lam, ldl, k, false);
+        }
         if (d_dA)
             launch_adjoint_dA<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), lam, ldl,
                                  static_cast<const V*>(d_x), ldx, k, conjugate,

@@ -150,6 +150,13 @@ cliqg_status cliqg_plan_solve_nodal(cliqg_plan plan, void* d_x, int ldx, int k);
 cliqg_status cliqg_plan_multiply(cliqg_plan plan, double alpha, const void* d_x, int ldx,
                                  double beta, void* d_y, int ldy, int k);
 
+/* Standalone FP64 sparse product y := alpha A x + beta y on device CSR
+ * arrays (original ordering), for Multiply without a plan. stream may be
+ * NULL (the null stream); the call is asynchronous on it. */
+cliqg_status cliqg_spmv(int N, const int64_t* d_row_off, const int* d_cols, const void* d_vals,
+                        cliqg_field field, double alpha, const void* d_x, int ldx, double beta,
+                        void* d_y, int ldy, int k, void* hip_stream);
+
 /* FP64 iterative refinement (no reference counterpart): x := solve(b); while
  * some column has backward error > tol and iterations < max_iters:
  * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified. */

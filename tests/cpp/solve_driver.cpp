@@ -1,0 +1,113 @@
+// Drop-in driver in the shape of the reference's tests/Solve.cpp (3D 7-point
+// operator, random X, Y := A X, natural nested dissection, LDL, Pull, Solve,
+// Push, error norms), written against the cliq:: surface of clique.hpp.
+// Exits non-zero if the relative error or the backward error is too large.
+//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [complex]
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "clique.hpp"
+using namespace cliq;
+
+template <typename F>
+int run(int n1, int n2, int n3, int numRhs, bool fp32) {
+    mpi::Comm comm = mpi::COMM_WORLD;
+    const int N = n1 * n2 * n3;
+    DistSparseMatrix<F> A(N, comm);
+    const F shift = IsComplex<F>::val ? F(6.0) + std::sqrt(F(-0.01)) : F(6.05);
+    A.StartAssembly();
+    A.Reserve(7 * N);
+    for (int i = 0; i < N; ++i) {
+        const int x = i % n1, y = (i / n1) % n2, z = i / (n1 * n2);
+        A.Update(i, i, shift);
+        if (x != 0) A.Update(i, i - 1, F(-1));
+        if (x != n1 - 1) A.Update(i, i + 1, F(-1));
+        if (y != 0) A.Update(i, i - n1, F(-1));
+        if (y != n2 - 1) A.Update(i, i + n1, F(-1));
+        if (z != 0) A.Update(i, i - n1 * n2, F(-1));
+        if (z != n3 - 1) A.Update(i, i + n1 * n2, F(-1));
+    }
+    A.StopAssembly();
+
+    DistMultiVec<F> X(N, numRhs, comm), Y(N, numRhs, comm);
+    MakeUniform(X);
+    Zero(Y);
+    Multiply(F(1), A, X, F(0), Y);
+    std::vector<typename Base<F>::type> YOrigNorms;
+    Norms(Y, YOrigNorms);
+
+    DistSymmInfo info;
+    DistSeparatorTree sepTree;
+    DistMap map, inverseMap;
+    NaturalNestedDissection(n1, n2, n3, A.DistGraph(), map, sepTree, info, 128);
+    map.FormInverse(inverseMap);
+    std::printf("%d local nodes, root separator %d\n", (int)info.localNodes.size(),
+                info.distNodes.back().size);
+
+    DistSymmFrontTree<F> frontTree(A, map, sepTree, info, false);
+    double lf, minf, maxf, gf;
+    frontTree.FactorizationWork(lf, minf, maxf, gf);
+    const double t0 = mpi::Time();
+    LDL(info, frontTree, LDL_1D, fp32 ? FACTOR_FP32 : FACTOR_FP64);
+    mpi::Barrier(comm);
+    const double t1 = mpi::Time();
+    std::printf("LDL: %.4f s, %.2f GFlop/s (device %.3f ms)\n", t1 - t0, gf / (1e9 * (t1 - t0)),
+                frontTree.LastFactorMilliseconds());
+
+    DistMultiVec<F> Xc(N, numRhs, comm);
+    double relErr = 0;
+    if (!fp32) {
+        DistNodalMultiVec<F> YNodal;
+        YNodal.Pull(inverseMap, info, Y);
+        Solve(info, frontTree, YNodal);
+        YNodal.Push(inverseMap, info, Xc);
+    } else {
+        cliqg_refine_info r = Refine(frontTree, Y, Xc, 1e-10, 30);
+        std::printf("refinement: %d iterations, backward error %.3e\n", r.iterations,
+                    r.max_backward_error);
+    }
+    std::vector<typename Base<F>::type> XNorms, errNorms;
+    Norms(X, XNorms);
+    Axpy(F(-1), X, Xc);
+    Norms(Xc, errNorms);
+    for (int j = 0; j < numRhs; ++j) {
+        const double e = errNorms[j] / XNorms[j];
+        relErr = std::max(relErr, e);
+        std::printf("rhs %d: ||x|| = %.6e  ||A x|| = %.6e  ||error||/||x|| = %.3e\n", j,
+                    (double)XNorms[j], (double)YOrigNorms[j], e);
+    }
+    // misuse is reported like the reference does
+    bool threw = false;
+    try {
+        LDL(info, frontTree, LDL_1D);
+    } catch (const LogicError&) {
+        threw = true;
+    }
+    const double tol = fp32 ? 1e-8 : 1e-10;
+    if (relErr > tol || !threw) {
+        std::printf("FAILED (relative error %.3e, refactor guard %d)\n", relErr, (int)threw);
+        return 1;
+    }
+    std::printf("PASSED\n");
+    return 0;
+}
+
+int main(int argc, char* argv[]) {
+    Initialize(argc, argv);
+    const int n1 = argc > 1 ? std::atoi(argv[1]) : 30;
+    const int n2 = argc > 2 ? std::atoi(argv[2]) : 30;
+    const int n3 = argc > 3 ? std::atoi(argv[3]) : 30;
+    const int k = argc > 4 ? std::atoi(argv[4]) : 5;
+    const bool fp32 = argc > 5 && std::string(argv[5]) == "fp32";
+    const bool cpx = argc > 6 && std::string(argv[6]) == "complex";
+    int rc = 1;
+    try {
+        rc = cpx ? run<std::complex<double>>(n1, n2, n3, k, fp32) : run<double>(n1, n2, n3, k, fp32);
+    } catch (const std::exception& e) {
+        std::printf("exception: %s\n", e.what());
+        rc = 2;
+    }
+    Finalize();
+    return rc;
+}
