@@ -45,6 +45,24 @@ def workload(cfg, n_override=None):
         b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
         desc = f"{n}^3 complex-symmetric Helmholtz (omega 2pi*6), natural ND cutoff 128, 4 RHS"
         return ip, ind, v, dims, b, True, desc
+    if cfg == 3:
+        import scipy.sparse as sp
+        count = n_override or 512
+        sizes = rng.integers(32, 193, size=(count, 2))
+        shifts = rng.uniform(0.01, 0.5, size=count)
+        mats, dims_list = [], []
+        for (nx, ny), s_ in zip(sizes, shifts):
+            ip_, ind_, v_, d_ = P.laplacian5(int(nx), int(ny), float(s_))
+            mats.append(sp.csr_matrix((v_, ind_, ip_)))
+            dims_list.append(d_)
+        A = sp.block_diag(mats, format="csr")
+        A.sort_indices()
+        N = A.shape[0]
+        b = rng.standard_normal((N, 16))
+        desc = (f"{count} independent 2D 5-point problems, nx,ny ~ U{{32..192}}, diag 4+s, "
+                f"one forest (natural ND cutoff 128 each), 16 RHS")
+        return (A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, dims_list, b,
+                False, desc)
     if cfg == 4:
         n = n_override or 80
         ip, ind, v, dims = P.diffusion_grf(n, seed=0)
@@ -105,7 +123,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=5, choices=[1, 2, 4, 5])
+    ap.add_argument("--config", type=int, default=5, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--n", type=int, default=None, help="override the grid size")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-fp64", action="store_true")
@@ -124,7 +142,18 @@ def main():
     ip, ind, v, dims, b, cplx, desc = workload(args.config, args.n)
     N = ip.size - 1
     k = b.shape[1]
-    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
+    if isinstance(dims, list):
+        # a forest of independent problems: one symbolic per block, concatenated
+        syms, o, e = [], 0, 0
+        for d in dims:
+            n_i = int(np.prod(d))
+            sub_ip = ip[o:o + n_i + 1] - ip[o]
+            sub_ind = ind[ip[o]:ip[o + n_i]] - o
+            syms.append(cq.NaturalNestedDissection(*d, sub_ip, sub_ind, cutoff=128))
+            o += n_i
+        sym = cq.Symbolic.concat(syms)
+    else:
+        sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
     flops, solve_flops, entries = sym.work(cplx)
     dev = torch.device("cuda:0")
     vals = torch.tensor(v, device=dev)

@@ -345,37 +345,47 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 
 // ---------------------------------------------------------------------------
 // Diagonal block: unpivoted LDL^T of a b x b block (b <= NB) in FP64 (field
-// type) in LDS, and the inverse of its unit lower factor. One workgroup per
-// front, one barrier per column: step j reads the (unscaled) column j and
-// applies both the rank-1 Schur update of the trailing block,
-//     a(i,c) -= a(i,j) op(a(c,j)) / d_j          (j < c <= i),
-// and the matching row operation on the inverse,
-//     li(i,c) -= (a(i,j) / d_j) li(j,c)          (c <= j < i),
-// neither of which touches column j or row j. Column j is scaled by 1/d_j
-// once all steps are done.
+// type) and the inverse of its unit lower factor. One workgroup of 256
+// threads per front; thread (ti, tj) of a 16 x 16 grid keeps the 4 x 4 tile
+// M[4ti.., 4tj..] of a 64 x 64 register-resident matrix whose lower triangle
+// holds the Schur-updated block (its column j is the unscaled L column at
+// step j) and whose strict upper triangle holds the transpose of the inverse
+// being built, li(q, i) at M[i][q]. Step j broadcasts column j of M through
+// LDS (double buffered, one barrier per step):
+//     a(i,c)  -= a(i,j) op(a(c,j)) / d_j          (lower, c > j)
+//     li(q,i) -= (a(q,j) / d_j) li(j,i)           (upper, i <= j < q; li(j,j) = 1)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
                                                   bool herm, int* status) {
     using W = typename Scalar<T>::wide;
-    extern __shared__ unsigned char smem_raw[];
-    W (*a)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw);
-    W (*li)[NB + 1] = reinterpret_cast<W (*)[NB + 1]>(smem_raw + sizeof(W) * NB * (NB + 1));
-    W* dinv = reinterpret_cast<W*>(smem_raw + 2 * sizeof(W) * NB * (NB + 1));
+    __shared__ W col[2][NB];
+    __shared__ W dinv[NB];
 
     const DiagDesc<T> d = descs[blockIdx.x];
     const int b = d.b, tid = threadIdx.x;
-    for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
-        const int i = idx % NB, j = idx / NB;
-        W v = W{};
-        if (i < b && j < b && i >= j) v = cvt<W>(d.A11[i + (int64_t)j * d.lda]);
-        a[i][j] = v;
-        li[i][j] = (i == j) ? one<W>() : W{};
-    }
+    const int ti = tid >> 4, tj = tid & 15;
+    W M[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * ti + r, c = 4 * tj + q;
+            M[r][q] = (i < b && c < b && i >= c) ? cvt<W>(d.A11[i + (int64_t)c * d.lda]) : W{};
+        }
     W* Dout = reinterpret_cast<W*>(d.D);
     for (int j = 0; j < b; ++j) {
+        const int buf = j & 1;
+        if (tj == (j >> 2)) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (q == (j & 3)) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) col[buf][4 * ti + r] = M[r][q];
+                }
+        }
         __syncthreads();
-        W piv = a[j][j];
+        W piv = col[buf][j];
         if (herm) piv = real_part_only(piv);
         const W ip = one<W>() / piv;
         if (tid == 0) {
@@ -383,51 +393,51 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
             Dout[j] = piv;
             dinv[j] = ip;
         }
-        const int r = b - j - 1;          // rows below the pivot
-        const int nt = r * (r + 1) / 2;   // trailing lower entries
-        const int nl = r * (j + 1);       // inverse entries updated
-        for (int idx = tid; idx < nt + nl; idx += blockDim.x) {
-            if (idx < nt) {
-                // (ii, cc) with cc <= ii over the r x r trailing triangle
-                int cc = (int)((sqrtf(8.f * idx + 1.f) - 1.f) * 0.5f);
-                while (cc * (cc + 1) / 2 > idx) --cc;
-                while ((cc + 1) * (cc + 2) / 2 <= idx) ++cc;
-                const int ii = idx - cc * (cc + 1) / 2;   // 0..cc
-                // map (row-major packed) to ii >= cc ordering: row = j+1+cc, col = j+1+ii
-                const int row = j + 1 + cc, col = j + 1 + ii;
-                a[row][col] = a[row][col] - a[row][j] * ip * cj(a[col][j], herm);
-            } else {
-                const int q = idx - nt;
-                const int row = j + 1 + q / (j + 1), col = q % (j + 1);
-                li[row][col] = li[row][col] - a[row][j] * ip * li[j][col];
-            }
+        W ci[4], cc[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            ci[r] = col[buf][4 * ti + r];
+            cc[r] = col[buf][4 * tj + r];
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = 4 * ti + r, c = 4 * tj + q;
+                if (i >= c) {
+                    if (c > j) M[r][q] = M[r][q] - ci[r] * ip * cj(cc[q], herm);
+                } else if (i <= j && c > j) {
+                    const W lij = (i == j) ? one<W>() : ci[r];
+                    M[r][q] = M[r][q] - cc[q] * ip * lij;
+                }
+            }
     }
     __syncthreads();
-    for (int idx = tid; idx < b * b; idx += blockDim.x) {
-        const int i = idx % b, j = idx / b;
-        if (i > j) d.A11[i + (int64_t)j * d.lda] = cvt<T>(a[i][j] * dinv[j]);
-        else if (i == j) d.A11[i + (int64_t)j * d.lda] = one<T>();
-    }
-    for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
-        const int i = idx % NB, j = idx / NB;
-        d.Linv[i + j * NB] = cvt<T>(li[i][j]);
-    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * ti + r, c = 4 * tj + q;
+            // L11 (unit diagonal) back into the front
+            if (i < b && c < b && i >= c)
+                d.A11[i + (int64_t)c * d.lda] = (i == c) ? one<T>() : cvt<T>(M[r][q] * dinv[c]);
+            // inverse of the unit lower factor, li(c,i) = M[i][c] for i < c.
+            // Each Linv entry has one writer: the owner of M[i][i] writes the
+            // diagonal, the owner of M[i][c] (i < c) writes (c,i) and (i,c).
+            if (i == c) {
+                d.Linv[i + c * NB] = (i < b) ? one<T>() : zero<T>();
+            } else if (i < c) {
+                d.Linv[c + i * NB] = (c < b) ? cvt<T>(M[r][q]) : zero<T>();
+                d.Linv[i + c * NB] = zero<T>();
+            }
+        }
 }
 
 template <class T>
 void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
                      hipStream_t s) {
     if (ndesc <= 0) return;
-    using W = typename Scalar<T>::wide;
-    const size_t shm = 2 * sizeof(W) * NB * (NB + 1) + sizeof(W) * NB;
-    static bool attr_set = false;
-    if (!attr_set) {
-        CQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_diag_ldl<T>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-        attr_set = true;
-    }
-    hipLaunchKernelGGL(k_diag_ldl<T>, dim3(ndesc), dim3(256), shm, s, d_descs, herm, d_status);
+    hipLaunchKernelGGL(k_diag_ldl<T>, dim3(ndesc), dim3(256), 0, s, d_descs, herm, d_status);
 }
 
 // ---------------------------------------------------------------------------
@@ -473,56 +483,29 @@ void launch_push(const T* xn, const int* inv_map, V* x, int64_t ldx, int N, int 
 }
 
 template <class T>
-__global__ void k_copy_in(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
-                          int nseg, int nrows, T* __restrict__ wpool, const T* __restrict__ xn,
-                          int N, int k, const int* __restrict__ lowpool, bool backward) {
+__global__ void k_gather(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
+                         int nseg, int nrows, T* __restrict__ wpool, const T* __restrict__ xn,
+                         int N, int k, const int* __restrict__ lowpool) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nrows) return;
     const int si = find_segment(row_start, nseg, g);
     const SolveSeg sg = segs[si];
     const int i = g - row_start[si];
     T* w = wpool + sg.w * k;
-    if (i < sg.n) {
-        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = xn[sg.off + i + (int64_t)j * N];
-    } else if (backward) {
-        const int src = lowpool[sg.low + (i - sg.n)];
-        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = xn[src + (int64_t)j * N];
-    } else {
-        for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.m] = zero<T>();
-    }
+    const int src = lowpool[sg.low + i];
+    for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.u] = xn[src + (int64_t)j * N];
 }
 template <class T>
-void launch_copy_in(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
-                    const T* xn, int N, int k, const int* lowpool, bool backward, hipStream_t s) {
+void launch_gather(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
+                   const T* xn, int N, int k, const int* lowpool, hipStream_t s) {
     if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_copy_in<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs, row_start,
-                       nseg, nrows, wpool, xn, N, k, lowpool, backward);
-}
-
-template <class T>
-__global__ void k_copy_out(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
-                           int nseg, int nrows, const T* __restrict__ wpool, T* __restrict__ xn,
-                           int N, int k) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nrows) return;
-    const int si = find_segment(row_start, nseg, g);
-    const SolveSeg sg = segs[si];
-    const int i = g - row_start[si];
-    if (i >= sg.n) return;
-    const T* w = wpool + sg.w * k;
-    for (int j = 0; j < k; ++j) xn[sg.off + i + (int64_t)j * N] = w[i + (int64_t)j * sg.m];
-}
-template <class T>
-void launch_copy_out(const SolveSeg* segs, const int* row_start, int nseg, int nrows,
-                     const T* wpool, T* xn, int N, int k, hipStream_t s) {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_copy_out<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs,
-                       row_start, nseg, nrows, wpool, xn, N, k);
+    hipLaunchKernelGGL(k_gather<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs, row_start,
+                       nseg, nrows, wpool, xn, N, k, lowpool);
 }
 
 template <class T>
 __global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __restrict__ row_start,
-                               int nd, int nrows, T* __restrict__ wpool,
+                               int nd, int nrows, T* __restrict__ wpool, T* __restrict__ xn, int N,
                                const int* __restrict__ relpool, int k) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nrows) return;
@@ -530,19 +513,27 @@ __global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __
     const SolveExtend d = ds[c];
     const int i = g - row_start[c];
     const T* wc = wpool + d.wc * k;
-    T* wp = wpool + d.wp * k;
     const int I = relpool[d.rel + i];
+    T* dst;
+    int64_t ld;
+    if (I < d.np) {
+        dst = xn + d.offp + I;
+        ld = N;
+    } else {
+        dst = wpool + d.wp * k + (I - d.np);
+        ld = d.up;
+    }
     for (int j = 0; j < k; ++j) {
-        T* p = wp + I + (int64_t)j * d.mp;
-        *p = *p + wc[d.nc + i + (int64_t)j * d.mc];
+        T* p = dst + (int64_t)j * ld;
+        *p = *p + wc[i + (int64_t)j * d.uc];
     }
 }
 template <class T>
 void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
-                         T* wpool, const int* relpool, int k, hipStream_t s) {
+                         T* wpool, T* xn, int N, const int* relpool, int k, hipStream_t s) {
     if (nrows <= 0) return;
     hipLaunchKernelGGL(k_solve_extend<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, d,
-                       row_start, nd, nrows, wpool, relpool, k);
+                       row_start, nd, nrows, wpool, xn, N, relpool, k);
 }
 
 template <class T, class V>
@@ -730,11 +721,9 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
     template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \
-    template void launch_copy_in<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,   \
-                                    int, const int*, bool, hipStream_t);                        \
-    template void launch_copy_out<T>(const SolveSeg*, const int*, int, int, const T*, T*, int,  \
-                                     int, hipStream_t);                                         \
-    template void launch_solve_extend<T>(const SolveExtend*, const int*, int, int, T*,          \
+    template void launch_gather<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,    \
+                                   int, const int*, hipStream_t);                               \
+    template void launch_solve_extend<T>(const SolveExtend*, const int*, int, int, T*, T*, int, \
                                          const int*, int, hipStream_t);
 
 CQ_INST_T(float)

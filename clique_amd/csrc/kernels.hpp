@@ -65,19 +65,23 @@ struct DiagDesc {
     int col0;        // k, for error reports
 };
 
-// Per-front segment of a solve-workspace operation.
+// Per-front segment of a solve operation: the update part W_s (u x k,
+// ld u) of the front's solve workspace and its lowerStruct. The front's own
+// rows live directly in the nodal right-hand sides X (ld N).
 struct SolveSeg {
-    int64_t w;       // offset of W_s (elements, for k = 1; times k, ld = m)
+    int64_t w;       // offset of W_s in the workspace pool (per right-hand side)
     int64_t low;     // offset of lowerStruct in the index pool
-    int n, m, off, pad;
+    int u, pad;
 };
 
-// Child-to-parent extend-add of solve workspaces (lower_solve/local.hpp:83-104)
+// Child update rows added into the parent (lower_solve/local.hpp:83-104):
+// relative indices below np address the parent's rows of X (at offp), the
+// others the parent's W.
 struct SolveExtend {
-    int64_t wc;      // child workspace offset
-    int64_t wp;      // parent workspace offset
+    int64_t wc;      // child workspace offset (per right-hand side)
+    int64_t wp;      // parent workspace offset (per right-hand side)
     int64_t rel;     // offset of the child's relative indices
-    int nc, mc, uc, mp;
+    int uc, np, offp, up;
 };
 
 // A contiguous range to clear.
@@ -116,15 +120,13 @@ template <class T, class V>
 void launch_pull(const V* x, int64_t ldx, const int* inv_map, T* xn, int N, int k, hipStream_t s);
 template <class T, class V>
 void launch_push(const T* xn, const int* inv_map, V* x, int64_t ldx, int N, int k, hipStream_t s);
+// W_s := X(lowerStruct_s, :) for every segment (backward sweep)
 template <class T>
-void launch_copy_in(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
-                    const T* xn, int N, int k, const int* lowpool, bool backward, hipStream_t s);
-template <class T>
-void launch_copy_out(const SolveSeg* segs, const int* row_start, int nseg, int nrows,
-                     const T* wpool, T* xn, int N, int k, hipStream_t s);
+void launch_gather(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
+                   const T* xn, int N, int k, const int* lowpool, hipStream_t s);
 template <class T>
 void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
-                         T* wpool, const int* relpool, int k, hipStream_t s);
+                         T* wpool, T* xn, int N, const int* relpool, int k, hipStream_t s);
 template <class T, class V>
 void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream_t s);
 

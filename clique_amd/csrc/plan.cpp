@@ -107,7 +107,7 @@ struct IntervalAlloc {
     }
 };
 
-enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_COPYIN, S_COPYOUT, S_SEXTEND, S_ZERO };
+enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO };
 
 struct Step {
     int kind;
@@ -125,8 +125,7 @@ const char* step_name(int kind) {
         case 1: return "diag_ldl";
         case 2: return "gemm";
         case 7: return "solve_gemm";
-        case 3: return "solve_copy_in";
-        case 4: return "solve_copy_out";
+        case 3: return "solve_gather";
         case 5: return "solve_extend_add";
         case 6: return "zero_update";
         default: return "other";
@@ -326,6 +325,14 @@ struct Numeric : NumericBase {
         farena.finish();
     }
 
+    // Solve schedule for k right-hand sides. The front's own rows are the
+    // nodal right-hand sides X (ld N); W_s (u x k, ld u) holds its update rows.
+    // Forward, per level: clear W, extend-add child slot 0 then 1, then per
+    // 64-wide panel X_k := Linv_k X_k and X(k+b:n) -= L(k+b:n, k) X_k, then
+    // W -= L21 X_s in one product.
+    // Backward, per level from the root: W := X(lowerStruct), X_s -= L21^T W,
+    // then per panel from the last X_k := Linv_k^T X_k and
+    // X(0:k) -= L(k:k+b, 0:k)^T X_k.
     SolveSched& schedule(int k) {
         auto it = solves.find(k);
         if (it != solves.end()) return *it->second;
@@ -334,7 +341,10 @@ struct Numeric : NumericBase {
         S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
         S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
         T* wp = S.wpool.template as<T>();
+        T* xn = S.xn.template as<T>();
+        const int64_t N = P.N;
         auto Wp = [&](int s) { return wp + P.fronts[s].W * k; };
+        auto Xs = [&](int s) { return xn + P.fronts[s].off; };
         const int cA = P.conjugate ? G_CONJA : 0;
         for (int pass = 0; pass < 2; ++pass) {
             const bool bwd = pass == 1;
@@ -343,20 +353,22 @@ struct Numeric : NumericBase {
             for (int hh = 0; hh < P.numLevels; ++hh) {
                 const int h = bwd ? P.numLevels - 1 - hh : hh;
                 const auto& lv = P.levels[h];
-                std::vector<SolveSeg> segs;
-                std::vector<int> rs;
-                int rows = 0;
-                for (int s : lv) {
-                    const FrontLayout& f = P.fronts[s];
-                    segs.push_back(SolveSeg{f.W, f.low, f.n, f.m, f.off, 0});
-                    rs.push_back(rows);
-                    rows += f.m;
-                }
-                const size_t segOff = S.arena.add(segs), rsOff = S.arena.add(rs);
-                steps.push_back(Step{S_COPYIN, segOff, rsOff, (int)segs.size(), rows, bwd});
                 int maxn = 0;
                 for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
                 if (!bwd) {
+                    std::vector<ZeroSeg<T>> zs;
+                    std::vector<int> zst;
+                    int chunks = 0;
+                    for (int s : lv) {
+                        const int64_t n = (int64_t)P.fronts[s].u * k;
+                        if (n == 0) continue;
+                        zs.push_back(ZeroSeg<T>{Wp(s), n});
+                        zst.push_back(chunks);
+                        chunks += (int)((n + 1023) / 1024);
+                    }
+                    if (!zs.empty())
+                        steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
+                                             (int)zs.size(), chunks, false});
                     for (int slot = 0; slot < 2; ++slot) {
                         std::vector<SolveExtend> ds;
                         std::vector<int> st;
@@ -366,7 +378,7 @@ struct Numeric : NumericBase {
                             if (fp.nchild <= slot) continue;
                             const FrontLayout& fc = P.fronts[fp.child[slot]];
                             if (fc.u == 0) continue;
-                            ds.push_back(SolveExtend{fc.W, fp.W, fc.rel, fc.n, fc.m, fc.u, fp.m});
+                            ds.push_back(SolveExtend{fc.W, fp.W, fc.rel, fc.u, fp.n, fp.off, fp.u});
                             st.push_back(tot);
                             tot += fc.u;
                         }
@@ -380,26 +392,46 @@ struct Numeric : NumericBase {
                             const FrontLayout& f = P.fronts[s];
                             if (f.n <= k0) continue;
                             const int b = std::min(NB, f.n - k0);
-                            T* w = Wp(s);
-                            tr.push_back(GemmDesc<T>{Li(s, k0), w + k0, w + k0, nullptr, nullptr,
-                                                     NB, f.m, f.m, b, k, b, 0});
-                            if (f.m - k0 - b > 0)
-                                up.push_back(GemmDesc<T>{L(s) + (k0 + b) + (int64_t)k0 * f.m, w + k0,
-                                                         w + k0 + b, nullptr, nullptr, f.m, f.m, f.m,
-                                                         f.m - k0 - b, k, b, G_ACCUM | G_NEG});
+                            T* x = Xs(s) + k0;
+                            tr.push_back(GemmDesc<T>{Li(s, k0), x, x, nullptr, nullptr, NB, N, N, b,
+                                                     k, b, 0});
+                            const T* lcol = L(s) + (int64_t)k0 * f.m;
+                            if (f.n - k0 - b > 0)
+                                up.push_back(GemmDesc<T>{lcol + k0 + b, x, x + b, nullptr, nullptr,
+                                                         f.m, N, N, f.n - k0 - b, k, b,
+                                                         G_ACCUM | G_NEG});
                         }
                         sb.gemm(tr, true);
                         sb.gemm(up);
                     }
-                } else {
+                    // the front's update rows in one product with K = n
                     std::vector<GemmDesc<T>> g;
                     for (int s : lv) {
                         const FrontLayout& f = P.fronts[s];
                         if (f.u == 0 || f.n == 0) continue;
-                        T* w = Wp(s);
-                        g.push_back(GemmDesc<T>{L(s) + f.n, w + f.n, w, nullptr, nullptr, f.m, f.m,
-                                                f.m, f.n, k, f.u, G_TRANSA | G_ACCUM | G_NEG | cA});
+                        g.push_back(GemmDesc<T>{L(s) + f.n, Xs(s), Wp(s), nullptr, nullptr, f.m, N,
+                                                f.u, f.u, k, f.n, G_ACCUM | G_NEG});
                     }
+                    sb.gemm(g);
+                } else {
+                    std::vector<SolveSeg> segs;
+                    std::vector<int> rs;
+                    int rows = 0;
+                    std::vector<GemmDesc<T>> g;
+                    for (int s : lv) {
+                        const FrontLayout& f = P.fronts[s];
+                        if (f.u == 0) continue;
+                        segs.push_back(SolveSeg{f.W, f.low, f.u, 0});
+                        rs.push_back(rows);
+                        rows += f.u;
+                        if (f.n > 0)
+                            g.push_back(GemmDesc<T>{L(s) + f.n, Wp(s), Xs(s), nullptr, nullptr, f.m,
+                                                    f.u, N, f.n, k, f.u,
+                                                    G_TRANSA | G_ACCUM | G_NEG | cA});
+                    }
+                    if (!segs.empty())
+                        steps.push_back(Step{S_GATHER, S.arena.add(segs), S.arena.add(rs),
+                                             (int)segs.size(), rows, true});
                     sb.gemm(g);
                     const int maxq = (maxn + NB - 1) / NB;
                     for (int q = maxq - 1; q >= 0; --q) {
@@ -409,19 +441,18 @@ struct Numeric : NumericBase {
                             const FrontLayout& f = P.fronts[s];
                             if (f.n <= k0) continue;
                             const int b = std::min(NB, f.n - k0);
-                            T* w = Wp(s);
-                            tr.push_back(GemmDesc<T>{Li(s, k0), w + k0, w + k0, nullptr, nullptr,
-                                                     NB, f.m, f.m, b, k, b, G_TRANSA | cA});
+                            T* x = Xs(s) + k0;
+                            tr.push_back(GemmDesc<T>{Li(s, k0), x, x, nullptr, nullptr, NB, N, N, b,
+                                                     k, b, G_TRANSA | cA});
                             if (k0 > 0)
-                                up.push_back(GemmDesc<T>{L(s) + k0, w + k0, w, nullptr, nullptr, f.m,
-                                                         f.m, f.m, k0, k, b,
+                                up.push_back(GemmDesc<T>{L(s) + k0, x, Xs(s), nullptr, nullptr, f.m,
+                                                         N, N, k0, k, b,
                                                          G_TRANSA | G_ACCUM | G_NEG | cA});
                         }
                         sb.gemm(tr, true);
                         sb.gemm(up);
                     }
                 }
-                steps.push_back(Step{S_COPYOUT, segOff, rsOff, (int)segs.size(), rows, bwd});
             }
         }
         S.arena.finish();
@@ -455,18 +486,15 @@ struct Numeric : NumericBase {
                     launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
                                    s.variant, S ? 1 : 0, st);
                     break;
-                case S_COPYIN:
-                    launch_copy_in<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                      S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k, P.dLow.as<int>(),
-                                      s.backward, st);
-                    break;
-                case S_COPYOUT:
-                    launch_copy_out<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc,
-                                       s.nwork, S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k, st);
+                case S_GATHER:
+                    launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
+                                     S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k,
+                                     P.dLow.as<int>(), st);
                     break;
                 case S_SEXTEND:
                     launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
-                                           s.nwork, S->wpool.template as<T>(), P.dRel.as<int>(), k, st);
+                                           s.nwork, S->wpool.template as<T>(),
+                                           S->xn.template as<T>(), P.N, P.dRel.as<int>(), k, st);
                     break;
             }
             if (P.profiling) {
@@ -566,7 +594,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
         f.slot = -1;
         f.L = Lp; Lp += (int64_t)f.m * f.n;
         f.U = 0;
-        f.W = Wp; Wp += f.m;
+        f.W = Wp; Wp += f.u;
         f.Linv = Ip; Ip += (int64_t)((f.n + NB - 1) / NB) * NB * NB;
         f.low = (int64_t)lowAll.size();
         lowAll.insert(lowAll.end(), nd.lowerStruct.begin(), nd.lowerStruct.end());

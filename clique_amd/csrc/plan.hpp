@@ -38,7 +38,7 @@ struct FrontLayout {
     int64_t U;      // update pool offset
     int64_t rel;    // offset of relInds into the parent's front (length u)
     int64_t low;    // offset of lowerStruct (length u)
-    int64_t W;      // solve workspace row offset (prefix of m)
+    int64_t W;      // solve workspace row offset (prefix of u)
     int64_t Linv;   // inverse diagonal block pool offset
 };
 

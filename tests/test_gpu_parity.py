@@ -181,3 +181,34 @@ def test_zero_pivot_is_reported(device):
     with pytest.raises(cq.CliqgError) as e:
         ft.factor(torch.tensor(v, device="cuda:0"))
     assert e.value.code == 2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batched_2d_problems_one_forest(device, precision):
+    """Config-3 shape at small scale: independent 2D 5-point problems of
+    uneven sizes factored as one forest; per-problem backward error."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(7)
+    sizes = rng.integers(8, 41, size=(12, 2))
+    shifts = rng.uniform(0.01, 0.5, size=12)
+    syms, mats = [], []
+    for (nx, ny), s in zip(sizes, shifts):
+        ip, ind, v, dims = problem("lap2d", (int(nx), int(ny)), s=float(s))
+        syms.append(cq.NaturalNestedDissection(*dims, ip, ind, cutoff=16))
+        mats.append(sp.csr_matrix((v, ind, ip)))
+    A = sp.block_diag(mats, format="csr")
+    A.sort_indices()
+    sym = cq.Symbolic.concat(syms)
+    ft = cq.DistSymmFrontTree(sym, A.indptr, A.indices)
+    ft.factor(torch.tensor(A.data, device="cuda:0"), precision=precision)
+    N = A.shape[0]
+    b = rng.standard_normal((N, 16))
+    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=20)
+    xg = x.cpu().numpy()
+    o = 0
+    for m in mats:
+        n = m.shape[0]
+        eta = backward_error(m.indptr, m.indices, m.data, xg[o:o + n], b[o:o + n])
+        assert eta.max() <= 1e-10
+        o += n
