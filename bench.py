@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-block", action="store_true",
+                    help="skip the BLOCK_LDL (selective inversion) FP32+IR line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -185,14 +187,57 @@ def main():
           f"IR iterations {[i['iterations'] for i in infos]}, eta {infos[-1]['backward_error']:.2e}",
           file=sys.stderr, flush=True)
 
-    # profiled (untimed) FP32 factorization: per-kernel-family time and flops
+    # profiled (untimed) FP32 factorization and one solve: per-kernel-family
+    # time and flops
     p32.set_profiling(True)
     p32.factor(vals, precision="f32")
     prof32 = p32.profile()
+    p32.set_profiling(True)    # restart the counts for the solve
+    x32.copy_(bd)
+    p32.solve(x32, nodal=False)
+    prof32s = p32.profile()
     p32.set_profiling(False)
     mem32 = p32.memory()
     del p32
     torch.cuda.empty_cache()
+
+    # BLOCK_LDL fronts (inverted diagonal blocks, GEMM-only solves), FP32 + IR
+    blk = None
+    if not args.no_block:
+        pb = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+        xb = cq.colmajor(torch.empty_like(bd))
+
+        def stepb():
+            pb.factor(vals, precision="f32", front_type="block_ldl")
+            fms = pb.last_factor_ms()
+            _, info = pb.refine(bd, xb, tol=args.tol, max_iters=30)
+            return fms, info
+
+        for _ in range(args.warmup):
+            stepb()
+        torch.cuda.synchronize()
+        fb_ms, infob = [], []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fms, info = stepb()
+            fb_ms.append(fms)
+            infob.append(info)
+        torch.cuda.synchronize()
+        tb = (time.perf_counter() - t0) / args.steps
+        pb.set_profiling(True)
+        xb.copy_(bd)
+        pb.solve(xb, nodal=False)
+        profbs = pb.profile()
+        pb.set_profiling(False)
+        blk = {"time_to_tol_ms": tb * 1e3, "factor_ms": float(np.mean(fb_ms)),
+               "refine_iterations": infob[-1]["iterations"],
+               "backward_error": infob[-1]["backward_error"],
+               "solve_ms": sum(e["ms"] for e in profbs.values())}
+        print(f"# block fp32+IR: {tb * 1e3:.1f} ms/step, factor {np.mean(fb_ms):.1f} ms, "
+              f"solve {blk['solve_ms']:.1f} ms, IR iterations {[i['iterations'] for i in infob]}",
+              file=sys.stderr, flush=True)
+        del pb
+        torch.cuda.empty_cache()
 
     fp64 = None
     if not args.no_fp64:
@@ -230,10 +275,12 @@ def main():
 
     g = prof32["gemm"]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
-    breakdown = {name: {"launches": e["launches"], "ms": round(e["ms"], 3),
-                        "tflops": (e["flops"] / (e["ms"] * 1e-3) / 1e12) if e["flops"] else None,
-                        "gbps": (e["bytes"] / (e["ms"] * 1e-3) / 1e9) if e["bytes"] else None}
-                 for name, e in prof32.items()}
+    def families(prof):
+        return {name: {"launches": e["launches"], "ms": round(e["ms"], 3),
+                       "tflops": (e["flops"] / (e["ms"] * 1e-3) / 1e12) if e["flops"] else None,
+                       "gbps": (e["bytes"] / (e["ms"] * 1e-3) / 1e9) if e["bytes"] else None}
+                for name, e in prof.items()}
+    breakdown = families(prof32)
     fac32 = float(np.mean(f32_ms))
     out = {
         "metric": METRIC,
@@ -263,6 +310,8 @@ def main():
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / g["launches"],
                      "flops_per_launch": g["flops"] / g["launches"]},
         "breakdown_fp32_factor": breakdown,
+        "breakdown_fp32_solve": families(prof32s),
+        "block_ldl_fp32_ir": blk,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)

@@ -250,6 +250,9 @@ class DistSymmFrontTree:
         d = np.zeros(max(n, 1), dtype=dt)
         _lib.check(lib().cliqg_plan_get_front(self._h, s, L.ctypes.data_as(ctypes.c_void_p),
                                               d.ctypes.data_as(ctypes.c_void_p)))
+        if self.front_type == "block_ldl":
+            # block fronts: ATL holds inv(L11 D L11^T), ABL the original A21, no diagonal
+            return L, d[:0]
         return L, d[:n]
 
     def memory(self):

@@ -48,7 +48,6 @@ __device__ __forceinline__ void record_status(int* status, int code, int front, 
     }
 }
 
-constexpr int TK = 16;
 constexpr int LPAD = 16;
 
 // number of column-major lower-trapezoid tiles before tile column tj
@@ -56,10 +55,54 @@ __host__ __device__ __forceinline__ long long lower_prefix(long long tj, long lo
     return tj * tilesM - tj * (tj - 1) / 2;
 }
 
+// Workgroup w of a launch runs on XCD w % 8, and each XCD has its own L2:
+// give XCD x the x-th contiguous eighth of the launch's tiles, so the tiles
+// an XCD holds at one time are neighbours that share operand panels.
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+    const int q = n / 8, rem = n % 8, x = w % 8, l = w / 8;
+    return (x < rem ? x * (q + 1) : rem * (q + 1) + (x - rem) * q) + l;
+}
+
+// Tiles of one descriptor in column groups of GROUP tile columns, row-major
+// inside a group: consecutive tiles sweep GROUP columns of one tile row,
+// then the next row. Lower launches keep only tiles with ti >= tj.
+constexpr int GROUP = 8;
+__device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, int& ti, int& tj) {
+    if (!lower) {
+        const int g = t / (tm * GROUP), c0 = g * GROUP;
+        const int w = min(GROUP, tn - c0), tl = t - g * tm * GROUP;
+        ti = tl / w;
+        tj = c0 + tl % w;
+        return;
+    }
+    const int c = min(tm, tn);
+    int c0 = 0;
+    for (;;) {
+        const int w = min(GROUP, c - c0);
+        const int tri = w * (w + 1) / 2, cnt = tri + (tm - c0 - w) * w;
+        if (t < cnt) {
+            if (t < tri) {
+                int r = (int)((sqrtf(8.0f * t + 1.0f) - 1.0f) * 0.5f);
+                while (r > 0 && r * (r + 1) / 2 > t) --r;
+                while ((r + 1) * (r + 2) / 2 <= t) ++r;
+                ti = c0 + r;
+                tj = c0 + (t - r * (r + 1) / 2);
+            } else {
+                const int u = t - tri;
+                ti = c0 + w + u / w;
+                tj = c0 + u % w;
+            }
+            return;
+        }
+        t -= cnt;
+        c0 += GROUP;
+    }
+}
+
 }  // namespace
 
 int gemm_tile_m(int variant) { return variant == 0 ? 64 : 128; }
-int gemm_tile_n(int variant) { return variant == 2 ? 128 : 64; }
+int gemm_tile_n(int variant) { return (variant == 2 || variant == 3) ? 128 : 64; }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
@@ -81,43 +124,39 @@ int gemm_tiles(int M, int N, int flags, int variant) {
 // the next chunk is prefetched into registers while the current one feeds
 // the MFMAs. Complex products use four real MFMAs on planar (re, im) tiles.
 // ---------------------------------------------------------------------------
-template <class T, int TM, int TN, int ROLE>
-__global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ descs,
-                                              const int* __restrict__ tile_start, int ndesc) {
+template <class T, int TM, int TN, int ROLE, int TK, int WM = 2, int WN = 2>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __restrict__ descs,
+                                                       const int* __restrict__ tile_start,
+                                                       int ndesc) {
     using R = typename Scalar<T>::real;
     using W = typename Scalar<T>::wide;
     using A4 = typename Acc4<R>::type;
     constexpr bool CPX = Scalar<T>::complex;
     constexpr int NP = CPX ? 2 : 1;
-    constexpr int BM = TM / 32, BN = TN / 32;      // 16x16 blocks per wave
-    constexpr int LA = TM * TK / 256, LB = TN * TK / 256;
+    constexpr int NT = 64 * WM * WN;                       // threads
+    constexpr int WTM = TM / WM, WTN = TN / WN;            // wave tile
+    constexpr int BM = WTM / 16, BN = WTN / 16;            // 16x16 blocks per wave
+    constexpr int LA = TM * TK / NT, LB = TN * TK / NT;
     constexpr bool F64 = sizeof(R) == 8;
+    constexpr bool SCALED = ROLE == 2;   // every descriptor of the launch has scaleK
 
     __shared__ R As[2][NP][TK][TM + LPAD];
     __shared__ R Bs[2][NP][TK][TN + LPAD];
+    // The pivots of a chunk (scaleK) live in column TM of As, which is padding
+    // the fragments never read: a separate array would push the complex tiles
+    // past a quarter (complex float) or half (complex double) of the 160 KB
+    // LDS and cost one resident block per CU.
 
-    const int tile = blockIdx.x;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
     const int di = find_segment(tile_start, ndesc, tile);
     const GemmDesc<T> d = descs[di];
     const int t = tile - tile_start[di];
-    const int tilesM = (d.M + TM - 1) / TM;
     int ti, tj;
-    if (d.flags & G_LOWER) {
-        const double b2 = 2.0 * tilesM + 1.0;
-        int c = (int)floor((b2 - sqrt(fmax(b2 * b2 - 8.0 * t, 0.0))) * 0.5);
-        if (c < 0) c = 0;
-        while (c > 0 && lower_prefix(c, tilesM) > t) --c;
-        while (lower_prefix(c + 1, tilesM) <= t) ++c;
-        tj = c;
-        ti = c + (int)(t - lower_prefix(c, tilesM));
-    } else {
-        ti = t % tilesM;
-        tj = t / tilesM;
-    }
+    group_tile(t, (d.M + TM - 1) / TM, (d.N + TN - 1) / TN, d.flags & G_LOWER, ti, tj);
     const int row0 = ti * TM, col0 = tj * TN;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = wave & 1, wj = wave >> 1;
+    const int wi = wave % WM, wj = wave / WM;
     const bool transA = d.flags & G_TRANSA, transB = d.flags & G_TRANSB;
     const bool conjA = d.flags & G_CONJA, conjB = d.flags & G_CONJB;
     const W* scaleK = reinterpret_cast<const W*>(d.scaleK);
@@ -131,77 +170,134 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
             accI[a][b] = A4{0, 0, 0, 0};
         }
 
+    // Fixed per-thread load geometry: element r of the A chunk is
+    // (iA + r*dIA, kA + r*dKA) of the TM x TK tile, element r of the B chunk
+    // (kB + r*dKB, jB + r*dJB) of the TK x TN tile; the global addresses of a
+    // thread's elements differ by multiples of lda (ldb), and a chunk advances
+    // them by a fixed stride.
+    static_assert(NT % TM == 0 && NT % TN == 0 && NT % TK == 0, "load geometry");
+    int iA, kA, dIA, dKA, jB, kB, dJB, dKB;
+    if (!transA) { iA = tid % TM; kA = tid / TM; dIA = 0; dKA = NT / TM; }
+    else { kA = tid % TK; iA = tid / TK; dIA = NT / TK; dKA = 0; }
+    if (transB) { jB = tid % TN; kB = tid / TN; dJB = 0; dKB = NT / TN; }
+    else { kB = tid % TK; jB = tid / TK; dJB = NT / TK; dKB = 0; }
+    const int64_t offA = (int64_t)(transA ? dIA : dKA) * d.lda;
+    const int64_t offB = (int64_t)(transB ? dKB : dJB) * d.ldb;
+    const int64_t stepA = transA ? TK : (int64_t)TK * d.lda;
+    const int64_t stepB = transB ? (int64_t)TK * d.ldb : TK;
+    unsigned rowOK = 0, colOK = 0;   // bit r: element r's row (column) is inside the matrix
+#pragma unroll
+    for (int r = 0; r < LA; ++r) rowOK |= (row0 + iA + r * dIA < d.M) ? 1u << r : 0u;
+#pragma unroll
+    for (int r = 0; r < LB; ++r) colOK |= (col0 + jB + r * dJB < d.N) ? 1u << r : 0u;
+    const int ldsA = kA * (TM + LPAD) + iA, ldsAStep = dKA * (TM + LPAD) + dIA;
+    const int ldsB = kB * (TN + LPAD) + jB, ldsBStep = dKB * (TN + LPAD) + dJB;
+
     T ra[LA], rb[LB];
+    T rd = one<T>();
+    const T* pA = nullptr;
+    const T* pB = nullptr;
+    auto start_at = [&](int k0) {
+        pA = transA ? d.A + (k0 + kA) + (int64_t)(row0 + iA) * d.lda
+                    : d.A + (row0 + iA) + (int64_t)(k0 + kA) * d.lda;
+        pB = transB ? d.B + (col0 + jB) + (int64_t)(k0 + kB) * d.ldb
+                    : d.B + (k0 + kB) + (int64_t)(col0 + jB) * d.ldb;
+    };
     auto load = [&](int k0) {
+        if constexpr (SCALED)
+            if (tid < TK) {
+                rd = zero<T>();
+                if (k0 + tid < d.K)
+                    rd = (d.flags & G_INVK) ? cvt<T>(one<W>() / scaleK[k0 + tid])
+                                            : cvt<T>(scaleK[k0 + tid]);
+            }
+        const bool fullK = k0 + TK <= d.K;
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
-            const int e = tid + 256 * r;
-            int i, k;
-            if (!transA) { i = e % TM; k = e / TM; } else { k = e & 15; i = e >> 4; }
-            const int gi = row0 + i, gk = k0 + k;
             T v = zero<T>();
-            if (gi < d.M && gk < d.K) {
-                v = transA ? d.A[gk + (int64_t)gi * d.lda] : d.A[gi + (int64_t)gk * d.lda];
-                if (scaleK) v = v * cvt<T>(scaleK[gk]);
+            if (((rowOK >> r) & 1u) && (fullK || k0 + kA + r * dKA < d.K)) {
+                v = pA[r * offA];
                 if (conjA) v = conj_(v);
             }
             ra[r] = v;
         }
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
-            const int e = tid + 256 * r;
-            int j, k;
-            if (transB) { j = e % TN; k = e / TN; } else { k = e & 15; j = e >> 4; }
-            const int gj = col0 + j, gk = k0 + k;
             T v = zero<T>();
-            if (gj < d.N && gk < d.K) {
-                v = transB ? d.B[gj + (int64_t)gk * d.ldb] : d.B[gk + (int64_t)gj * d.ldb];
+            if (((colOK >> r) & 1u) && (fullK || k0 + kB + r * dKB < d.K)) {
+                v = pB[r * offB];
                 if (conjB) v = conj_(v);
             }
             rb[r] = v;
         }
+        pA += stepA;
+        pB += stepB;
     };
     auto store = [&](int buf) {
+        if constexpr (SCALED)
+            if (tid < TK) {
+                As[buf][0][tid][TM] = re(rd);
+                if constexpr (CPX) As[buf][NP - 1][tid][TM] = im(rd);
+            }
+        R* a0 = &As[buf][0][0][0];
+        R* a1 = &As[buf][NP - 1][0][0];
+        R* b0 = &Bs[buf][0][0][0];
+        R* b1 = &Bs[buf][NP - 1][0][0];
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
-            const int e = tid + 256 * r;
-            int i, k;
-            if (!transA) { i = e % TM; k = e / TM; } else { k = e & 15; i = e >> 4; }
-            As[buf][0][k][i] = re(ra[r]);
-            if constexpr (CPX) As[buf][NP - 1][k][i] = im(ra[r]);
+            a0[ldsA + r * ldsAStep] = re(ra[r]);
+            if constexpr (CPX) a1[ldsA + r * ldsAStep] = im(ra[r]);
         }
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
-            const int e = tid + 256 * r;
-            int j, k;
-            if (transB) { j = e % TN; k = e / TN; } else { k = e & 15; j = e >> 4; }
-            Bs[buf][0][k][j] = re(rb[r]);
-            if constexpr (CPX) Bs[buf][NP - 1][k][j] = im(rb[r]);
+            b0[ldsB + r * ldsBStep] = re(rb[r]);
+            if constexpr (CPX) b1[ldsB + r * ldsBStep] = im(rb[r]);
         }
     };
 
-    const int nk = (d.K + TK - 1) / TK;
+    // K range of this row tile: triangular op(A) bounds it by the tile's rows
+    int kbeg = 0, kend = d.K;
+    if (d.flags & G_KLO) kbeg = min((row0 / TK) * TK, d.K);
+    if (d.flags & G_KHI) kend = min(d.K, row0 + TM);
+    const int nk = kend > kbeg ? (kend - kbeg + TK - 1) / TK : 0;
+    start_at(kbeg);
     if (nk > 0) {
-        load(0);
+        load(kbeg);
         store(0);
         __syncthreads();
     }
     for (int c = 0; c < nk; ++c) {
         const int buf = c & 1;
-        if (c + 1 < nk) load((c + 1) * TK);
+        if (c + 1 < nk) load(kbeg + (c + 1) * TK);
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
             R aR[BM], aI[BM], bR[BN], bI[BN];
 #pragma unroll
             for (int q = 0; q < BM; ++q) {
-                aR[q] = As[buf][0][kr][wi * (TM / 2) + q * 16 + (lane & 15)];
-                if constexpr (CPX) aI[q] = As[buf][NP - 1][kr][wi * (TM / 2) + q * 16 + (lane & 15)];
+                aR[q] = As[buf][0][kr][wi * WTM + q * 16 + (lane & 15)];
+                if constexpr (CPX) aI[q] = As[buf][NP - 1][kr][wi * WTM + q * 16 + (lane & 15)];
+            }
+            if constexpr (SCALED) {
+                // op(A)(i,k) * D(k): the lane's k is kr for all of its A fragments
+                const R dR = As[buf][0][kr][TM];
+                const R dI = CPX ? As[buf][NP - 1][kr][TM] : R(0);
+#pragma unroll
+                for (int q = 0; q < BM; ++q) {
+                    if constexpr (CPX) {
+                        const R xr = aR[q] * dR - aI[q] * dI;
+                        const R xi = aR[q] * dI + aI[q] * dR;
+                        aR[q] = xr;
+                        aI[q] = xi;
+                    } else {
+                        aR[q] = aR[q] * dR;
+                    }
+                }
             }
 #pragma unroll
             for (int q = 0; q < BN; ++q) {
-                bR[q] = Bs[buf][0][kr][wj * (TN / 2) + q * 16 + (lane & 15)];
-                if constexpr (CPX) bI[q] = Bs[buf][NP - 1][kr][wj * (TN / 2) + q * 16 + (lane & 15)];
+                bR[q] = Bs[buf][0][kr][wj * WTN + q * 16 + (lane & 15)];
+                if constexpr (CPX) bI[q] = Bs[buf][NP - 1][kr][wj * WTN + q * 16 + (lane & 15)];
             }
 #pragma unroll
             for (int a = 0; a < BM; ++a)
@@ -221,6 +317,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
 
     const W* scaleN = reinterpret_cast<const W*>(d.scaleN);
     const bool lower = d.flags & G_LOWER, accum = d.flags & G_ACCUM, neg = d.flags & G_NEG;
+    const bool symm = lower && (d.flags & G_SYMM);
 #pragma unroll
     for (int a = 0; a < BM; ++a)
 #pragma unroll
@@ -228,25 +325,33 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc<T>* __restrict__ de
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int rin = F64 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
-                const int gi = row0 + wi * (TM / 2) + a * 16 + rin;
-                const int gj = col0 + wj * (TN / 2) + b * 16 + (lane & 15);
+                const int gi = row0 + wi * WTM + a * 16 + rin;
+                const int gj = col0 + wj * WTN + b * 16 + (lane & 15);
                 if (gi >= d.M || gj >= d.N || (lower && gi < gj)) continue;
                 T v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
                 if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
                 if (neg) v = -v;
                 T* p = d.C + gi + (int64_t)gj * d.ldc;
                 if (accum) v = *p + v;
+                if (symm) {
+                    // Hermitian output (conj mirror) has a real diagonal
+                    if (conjA && gi == gj) v = make_t<T>(re(v), R(0));
+                    if (gi > gj) d.C[gj + (int64_t)gi * d.ldc] = conjA ? conj_(v) : v;
+                }
                 *p = v;
             }
 }
 
-template <class T, int TM, int TN>
+template <class T, int TM, int TN, int KT = 16, int WM = 2, int WN = 2>
 static void launch_gemm_shape(const GemmDesc<T>* d, const int* ts, int nd, int nt, int role,
                               hipStream_t s) {
+    const dim3 block(64 * WM * WN);
     if (role == 0)
-        hipLaunchKernelGGL((k_gemm<T, TM, TN, 0>), dim3(nt), dim3(256), 0, s, d, ts, nd);
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 0, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
+    else if (role == 2)
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 2, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
     else
-        hipLaunchKernelGGL((k_gemm<T, TM, TN, 1>), dim3(nt), dim3(256), 0, s, d, ts, nd);
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 1, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
 }
 
 template <class T>
@@ -255,11 +360,16 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
     if (ntiles <= 0 || ndesc <= 0) return;
     if (variant == 2) {
         if constexpr (gemm_variant_allowed<T>(2))
-            launch_gemm_shape<T, 128, 128>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+            launch_gemm_shape<T, 128, 128, 16, 2, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
             throw Error(1, "128x128 GEMM tiles are not built for complex double");
     } else if (variant == 1) {
         launch_gemm_shape<T, 128, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+    } else if (variant == 3) {
+        if constexpr (gemm_variant_allowed<T>(3))
+            launch_gemm_shape<T, 128, 128, 16, 4, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+        else
+            throw Error(1, "128x128 GEMM tiles are not built for complex double");
     } else {
         launch_gemm_shape<T, 64, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     }
@@ -288,6 +398,29 @@ void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, in
     if (nchunks <= 0) return;
     hipLaunchKernelGGL(k_zero_segments<T>, dim3(nchunks), dim3(256), 0, s, d_segs, d_chunk_start,
                        nseg);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_copy2d(const CopyDesc<T>* __restrict__ descs,
+                                                const int* __restrict__ chunk_start, int ndesc) {
+    const int g = blockIdx.x;
+    const int di = find_segment(chunk_start, ndesc, g);
+    const CopyDesc<T> d = descs[di];
+    const int64_t total = (int64_t)d.rows * d.cols;
+    const int64_t base = (int64_t)(g - chunk_start[di]) * 1024;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+        const int64_t e = base + i;
+        if (e >= total) break;
+        const int64_t r = e % d.rows, c = e / d.rows;
+        d.dst[r + c * d.ldd] = d.src[r + c * d.lds];
+    }
+}
+
+template <class T>
+void launch_copy2d(const CopyDesc<T>* d_descs, const int* d_chunk_start, int ndesc, int nchunks,
+                   hipStream_t s) {
+    if (nchunks <= 0 || ndesc <= 0) return;
+    hipLaunchKernelGGL(k_copy2d<T>, dim3(nchunks), dim3(256), 0, s, d_descs, d_chunk_start, ndesc);
 }
 
 // ---------------------------------------------------------------------------
@@ -361,6 +494,7 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
     using W = typename Scalar<T>::wide;
     __shared__ W col[2][NB];
     __shared__ W dinv[NB];
+    __shared__ W dval[NB];
 
     const DiagDesc<T> d = descs[blockIdx.x];
     const int b = d.b, tid = threadIdx.x;
@@ -388,9 +522,10 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
         W piv = col[buf][j];
         if (herm) piv = real_part_only(piv);
         const W ip = one<W>() / piv;
+        // pivots stay in LDS during the loop: a global store here would make
+        // every barrier wait for it (the barrier's fence covers global memory)
         if (tid == 0) {
-            if (is_zero_or_nonfinite(piv)) record_status(status, 2, d.front, d.col0 + j);
-            Dout[j] = piv;
+            dval[j] = piv;
             dinv[j] = ip;
         }
         W ci[4], cc[4];
@@ -413,6 +548,10 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
             }
     }
     __syncthreads();
+    for (int j = tid; j < b; j += blockDim.x) {
+        Dout[j] = dval[j];
+        if (is_zero_or_nonfinite(dval[j])) record_status(status, 2, d.front, d.col0 + j);
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
@@ -718,6 +857,7 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
 // ---------------------------------------------------------------------------
 #define CQ_INST_T(T)                                                                           \
     template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
+    template void launch_copy2d<T>(const CopyDesc<T>*, const int*, int, int, hipStream_t);       \
     template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \

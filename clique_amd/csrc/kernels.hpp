@@ -10,8 +10,9 @@ namespace cliqg {
 
 // Width of a diagonal block (panel) of the partial LDL^T.
 constexpr int NB = 64;
-// Tile shapes of the grouped GEMM: variant 0 = 64x64, 1 = 128x64,
-// 2 = 128x128 (not for complex double: its accumulators would not fit).
+// Tile shapes of the grouped GEMM: variant 0 = 64x64 (4 waves), 1 = 128x64
+// (4 waves), 2 = 128x128 (8 waves), 3 = 128x128 (16 waves); 2 and 3 are not
+// built for complex double (LDS).
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
 template <class T>
@@ -27,9 +28,14 @@ enum GemmFlags : int {
     G_LOWER = 16,   // compute/write only C(i,j) with i >= j
     G_ACCUM = 32,   // C := C + alpha*op(A)op(B)  (else C := alpha*op(A)op(B))
     G_NEG = 64,     // alpha = -1 (else +1)
+    G_INVK = 128,   // scale op(A)'s column k by 1/scaleK[k] instead of scaleK[k]
+    G_KLO = 256,    // op(A)(i,k) = 0 for k < i: a row tile starts K at its first row
+    G_KHI = 512,    // op(A)(i,k) = 0 for k > i: a row tile stops K after its last row
+    G_SYMM = 1024,  // with G_LOWER: also write C(j,i) = C(i,j) (conjugated with G_CONJA)
 };
 
-// C(MxN) (+)= +-op(A)(MxK) diag(scaleK) op(B)(KxN) [diag(scaleN)^{-1}]
+// C(MxN) (+)= +-op(A)(MxK) diag(scaleK) op(B)(KxN) [diag(scaleN)^{-1}], where
+// op(A) is A, A^T, conj(A) or A^H per the flags (scaleK is never conjugated).
 template <class T>
 struct GemmDesc {
     const T* A;
@@ -40,6 +46,15 @@ struct GemmDesc {
     int64_t lda, ldb, ldc;
     int M, N, K;
     int flags;
+};
+
+// Dense copy dst(0:rows, 0:cols) := src(0:rows, 0:cols), both column-major.
+template <class T>
+struct CopyDesc {
+    const T* src;
+    T* dst;
+    int64_t lds, ldd;
+    int rows, cols;
 };
 
 // Child-to-parent extend-add of one child's update matrix
@@ -97,9 +112,15 @@ struct ZeroSeg {
 template <class T>
 void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, int nseg,
                           int nchunks, hipStream_t s);
+// role: 0 factor, 1 solve, 2 factor with every descriptor scaled by scaleK
+// (separate instantiations keep the unscaled kernels' register budget).
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
                  int variant, int role, hipStream_t s);
+// chunk_start[i] = number of 1024-element chunks before descriptor i
+template <class T>
+void launch_copy2d(const CopyDesc<T>* d_descs, const int* d_chunk_start, int ndesc, int nchunks,
+                   hipStream_t s);
 // number of output tiles of a descriptor (host side, must match the kernel)
 int gemm_tiles(int M, int N, int flags, int variant);
 

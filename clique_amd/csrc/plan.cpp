@@ -107,7 +107,7 @@ struct IntervalAlloc {
     }
 };
 
-enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO };
+enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY };
 
 struct Step {
     int kind;
@@ -117,6 +117,7 @@ struct Step {
     double flops = 0.0;   // algorithmic flops of the launch (reference convention)
     double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
     int variant = 0;      // GEMM tile shape
+    bool scaled = false;  // GEMM descriptors carry pivots (scaleK)
 };
 
 const char* step_name(int kind) {
@@ -128,6 +129,7 @@ const char* step_name(int kind) {
         case 3: return "solve_gather";
         case 5: return "solve_extend_add";
         case 6: return "zero_update";
+        case 8: return "copy";
         default: return "other";
     }
 }
@@ -137,8 +139,42 @@ struct StepBuilder {
     Arena& arena;
     std::vector<Step>& steps;
 
-    // triB: op(B) is a triangular K x K block (the inverted diagonal blocks),
-    // so only half of the K products count as algorithmic work.
+    void zero(const std::vector<ZeroSeg<T>>& zs) {
+        std::vector<int> start;
+        int chunks = 0;
+        std::vector<ZeroSeg<T>> keep;
+        for (const auto& z : zs) {
+            if (z.n <= 0) continue;
+            keep.push_back(z);
+            start.push_back(chunks);
+            chunks += (int)((z.n + 1023) / 1024);
+        }
+        if (keep.empty()) return;
+        steps.push_back(Step{S_ZERO, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
+                             false});
+    }
+
+    void copy(const std::vector<CopyDesc<T>>& cs) {
+        std::vector<int> start;
+        int chunks = 0;
+        double by = 0.0;
+        std::vector<CopyDesc<T>> keep;
+        for (const auto& c : cs) {
+            const int64_t e = (int64_t)c.rows * c.cols;
+            if (e <= 0) continue;
+            keep.push_back(c);
+            start.push_back(chunks);
+            chunks += (int)((e + 1023) / 1024);
+            by += 2.0 * e * sizeof(T);
+        }
+        if (keep.empty()) return;
+        steps.push_back(Step{S_COPY, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
+                             false, 0.0, by});
+    }
+
+    // triB: one operand is triangular (an inverted diagonal block, or a
+    // triangular op(A) whose K range the kernel clamps), so only half of the
+    // K products count as algorithmic work.
     void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) {
         std::vector<GemmDesc<T>> ds;
         std::vector<int> start;
@@ -159,8 +195,13 @@ struct StepBuilder {
             bytes += (M * K + K * N + entries * ((d.flags & G_ACCUM) ? 2 : 1)) * sizeof(T);
         }
         if (ds.empty()) return;
+        const bool scaled = ds[0].scaleK != nullptr;
+        for (const auto& d : ds)
+            if ((d.scaleK != nullptr) != scaled)
+                throw Error(1, "a grouped GEMM mixes scaled and unscaled descriptors");
         Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false, flops, bytes};
         s.variant = variant;
+        s.scaled = scaled;
         steps.push_back(s);
     }
 
@@ -202,13 +243,17 @@ struct Numeric : NumericBase {
     using V = typename Scalar<T>::wide;
     Plan& P;
     DevBuf fpool, upool, dpool, linv;
-    Arena farena;
-    std::vector<Step> fsteps;
+    Arena farena, barena;
+    std::vector<Step> fsteps, bsteps;   // LDL and block-LDL factor schedules
+    bool blockBuilt = false;
+    // block LDL work buffers, reused level by level: the fronts' original
+    // A21, the unit-lower inverses of L11 and the n x 64 product panels
+    DevBuf bsave, bsinv, btmp;
 
     struct SolveSched {
         Arena arena;
         std::vector<Step> fwd, bwd;
-        DevBuf wpool, xn;
+        DevBuf wpool, xn, tmp;
     };
     std::map<int, std::unique_ptr<SolveSched>> solves;
     DevBuf rbuf, identity;
@@ -219,14 +264,18 @@ struct Numeric : NumericBase {
         upool.alloc(std::max<int64_t>(P.updateEntries, 1) * sizeof(T));
         dpool.alloc(std::max(P.N, 1) * sizeof(V));
         linv.alloc(std::max<int64_t>(P.linvEntries, 1) * sizeof(T));
-        build_factor_schedule();
+        build_factor_schedule(farena, fsteps, false);
     }
+
+    bool block() const { return P.frontType == 1; }
 
     size_t device_bytes() const override {
         size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
-                   rbuf.bytes + identity.bytes;
+                   barena.dev.bytes + bsave.bytes + bsinv.bytes + btmp.bytes + rbuf.bytes +
+                   identity.bytes;
         for (auto& kv : solves)
-            b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes;
+            b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
+                 kv.second->tmp.bytes;
         return b;
     }
 
@@ -235,9 +284,38 @@ struct Numeric : NumericBase {
     V* D(int s) const { return dpool.as<V>() + P.fronts[s].off; }
     T* Li(int s, int k0) const { return linv.as<T>() + P.fronts[s].Linv + (int64_t)(k0 / NB) * NB * NB; }
 
-    void build_factor_schedule() {
-        StepBuilder<T> sb{farena, fsteps};
+    // Factor schedule. Block LDL (ldl/local_front_block.hpp:25-73) adds, per
+    // level: save A21 after the extend-add; after the Schur update, the
+    // unit-lower inverse X = L11^{-1} built block column by block column from
+    // the last (X21_j = -(X22 L21_j) Linv_jj, X_jj = Linv_jj) in a separate
+    // n x n buffer; ATL := X^T D^{-1} X (X^H for Hermitian) written to both
+    // triangles; and A21 restored.
+    void build_factor_schedule(Arena& arena, std::vector<Step>& steps, bool blockLdl) {
+        StepBuilder<T> sb{arena, steps};
         const int herm = P.conjugate ? G_CONJB : 0;
+        std::vector<int64_t> saveOff(P.numFronts, 0), sinvOff(P.numFronts, 0),
+            tmpOff(P.numFronts, 0);
+        if (blockLdl) {
+            int64_t maxSave = 0, maxSinv = 0, maxTmp = 0;
+            for (int h = 0; h < P.numLevels; ++h) {
+                int64_t a = 0, b = 0, c = 0;
+                for (int s : P.levels[h]) {
+                    const FrontLayout& f = P.fronts[s];
+                    saveOff[s] = a; a += (int64_t)f.u * f.n;
+                    sinvOff[s] = b; b += (int64_t)f.n * f.n;
+                    tmpOff[s] = c; c += (int64_t)f.n * NB;
+                }
+                maxSave = std::max(maxSave, a);
+                maxSinv = std::max(maxSinv, b);
+                maxTmp = std::max(maxTmp, c);
+            }
+            bsave.alloc(std::max<int64_t>(maxSave, 1) * sizeof(T));
+            bsinv.alloc(std::max<int64_t>(maxSinv, 1) * sizeof(T));
+            btmp.alloc(std::max<int64_t>(maxTmp, 1) * sizeof(T));
+        }
+        auto Sv = [&](int s) { return bsave.as<T>() + saveOff[s]; };
+        auto Xi = [&](int s) { return bsinv.as<T>() + sinvOff[s]; };
+        auto Tp = [&](int s) { return btmp.as<T>() + tmpOff[s]; };
         for (int h = 0; h < P.numLevels; ++h) {
             const auto& lv = P.levels[h];
             {
@@ -253,7 +331,7 @@ struct Numeric : NumericBase {
                     chunks += (int)((n + 1023) / 1024);
                 }
                 if (!zs.empty())
-                    fsteps.push_back(Step{S_ZERO, farena.add(zs), farena.add(start), (int)zs.size(),
+                    steps.push_back(Step{S_ZERO, arena.add(zs), arena.add(start), (int)zs.size(),
                                           chunks, false});
             }
             for (int slot = 0; slot < 2; ++slot) {
@@ -274,9 +352,17 @@ struct Numeric : NumericBase {
                 if (!ds.empty()) {
                     double by = 0.0;
                     for (const auto& x : ds) by += 3.0 * x.uc * (x.uc + 1.0) / 2.0 * sizeof(T);
-                    fsteps.push_back(Step{S_EXTEND, farena.add(ds), farena.add(start),
+                    steps.push_back(Step{S_EXTEND, arena.add(ds), arena.add(start),
                                           (int)ds.size(), total, false, 0.0, by});
                 }
+            }
+            if (blockLdl) {
+                std::vector<CopyDesc<T>> cs;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    cs.push_back(CopyDesc<T>{L(s) + f.n, Sv(s), f.m, f.u, f.u, f.n});
+                }
+                sb.copy(cs);
             }
             int maxn = 0;
             for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
@@ -306,7 +392,7 @@ struct Numeric : NumericBase {
                 if (!dd.empty()) {
                     double fl = 0.0;
                     for (const auto& x : dd) fl += (double)x.b * x.b * x.b / 3.0;
-                    fsteps.push_back(Step{S_DIAG, farena.add(dd), 0, (int)dd.size(), 0, false,
+                    steps.push_back(Step{S_DIAG, arena.add(dd), 0, (int)dd.size(), 0, false,
                                           fl * (Scalar<T>::complex ? 4 : 1), 0.0});
                 }
                 sb.gemm(trsm, true);
@@ -321,8 +407,57 @@ struct Numeric : NumericBase {
                                            f.n, G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
             }
             sb.gemm(syrk);
+            if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp);
         }
-        farena.finish();
+        arena.finish();
+    }
+
+    template <class FS, class FX, class FT>
+    void block_inverse(StepBuilder<T>& sb, const std::vector<int>& lv, FS Sv, FX Xi, FT Tp) {
+        const int cA = P.conjugate ? G_CONJA : 0;
+        std::vector<ZeroSeg<T>> zs;
+        int maxq = 0;
+        for (int s : lv) {
+            const FrontLayout& f = P.fronts[s];
+            zs.push_back(ZeroSeg<T>{Xi(s), (int64_t)f.n * f.n});
+            maxq = std::max(maxq, (f.n + NB - 1) / NB);
+        }
+        sb.zero(zs);
+        for (int t = 0; t < maxq; ++t) {
+            std::vector<CopyDesc<T>> diagBlocks;
+            std::vector<GemmDesc<T>> trmm, scal;
+            for (int s : lv) {
+                const FrontLayout& f = P.fronts[s];
+                const int q = (f.n + NB - 1) / NB;
+                if (q <= t) continue;
+                const int j0 = (q - 1 - t) * NB, b = std::min(NB, f.n - j0), r = f.n - j0 - b;
+                const int64_t n = f.n;
+                T* X = Xi(s);
+                diagBlocks.push_back(CopyDesc<T>{Li(s, j0), X + j0 + j0 * n, NB, n, b, b});
+                if (r > 0) {
+                    T* x22 = X + (j0 + b) + (j0 + b) * n;
+                    const T* l21 = L(s) + (j0 + b) + (int64_t)j0 * f.m;
+                    trmm.push_back(GemmDesc<T>{x22, l21, Tp(s), nullptr, nullptr, n, f.m, n, r, b,
+                                               r, G_KHI});
+                    scal.push_back(GemmDesc<T>{Tp(s), Li(s, j0), X + (j0 + b) + j0 * n, nullptr,
+                                               nullptr, n, NB, n, r, b, b, G_NEG});
+                }
+            }
+            sb.copy(diagBlocks);
+            sb.gemm(trmm, true);
+            sb.gemm(scal, true);
+        }
+        std::vector<GemmDesc<T>> inv;
+        std::vector<CopyDesc<T>> restore;
+        for (int s : lv) {
+            const FrontLayout& f = P.fronts[s];
+            if (f.n == 0) continue;
+            inv.push_back(GemmDesc<T>{Xi(s), Xi(s), L(s), D(s), nullptr, f.n, f.n, f.m, f.n, f.n,
+                                      f.n, G_TRANSA | G_LOWER | G_SYMM | G_INVK | G_KLO | cA});
+            restore.push_back(CopyDesc<T>{Sv(s), L(s) + f.n, f.u, f.m, f.u, f.n});
+        }
+        sb.gemm(inv, true);
+        sb.copy(restore);
     }
 
     // Solve schedule for k right-hand sides. The front's own rows are the
@@ -334,14 +469,31 @@ struct Numeric : NumericBase {
     // then per panel from the last X_k := Linv_k^T X_k and
     // X(0:k) -= L(k:k+b, 0:k)^T X_k.
     SolveSched& schedule(int k) {
-        auto it = solves.find(k);
+        const bool blk = block();
+        const int key = 2 * k + (blk ? 1 : 0);
+        auto it = solves.find(key);
         if (it != solves.end()) return *it->second;
         auto sp = std::make_unique<SolveSched>();
         SolveSched& S = *sp;
         S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
         S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
+        std::vector<int64_t> tmpOff(P.numFronts, 0);
+        if (blk) {
+            int64_t mx = 0;
+            for (int h = 0; h < P.numLevels; ++h) {
+                int64_t a = 0;
+                for (int s : P.levels[h]) {
+                    tmpOff[s] = a;
+                    a += (int64_t)P.fronts[s].n * k;
+                }
+                mx = std::max(mx, a);
+            }
+            S.tmp.alloc(std::max<int64_t>(mx, 1) * sizeof(T));
+        }
         T* wp = S.wpool.template as<T>();
         T* xn = S.xn.template as<T>();
+        T* tp = S.tmp.template as<T>();
+        auto Tm = [&](int s) { return tp + tmpOff[s]; };
         const int64_t N = P.N;
         auto Wp = [&](int s) { return wp + P.fronts[s].W * k; };
         auto Xs = [&](int s) { return xn + P.fronts[s].off; };
@@ -386,7 +538,22 @@ struct Numeric : NumericBase {
                             steps.push_back(Step{S_SEXTEND, S.arena.add(ds), S.arena.add(st),
                                                  (int)ds.size(), tot, false});
                     }
-                    for (int k0 = 0; k0 < maxn; k0 += NB) {
+                    if (blk) {
+                        // FrontBlockLowerForwardSolve (lower_solve/local_front_block.hpp:28-48):
+                        // X_s := inv(ATL) X_s through a copy of X_s
+                        std::vector<CopyDesc<T>> cs;
+                        std::vector<GemmDesc<T>> g;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.n == 0) continue;
+                            cs.push_back(CopyDesc<T>{Xs(s), Tm(s), N, f.n, f.n, k});
+                            g.push_back(GemmDesc<T>{L(s), Tm(s), Xs(s), nullptr, nullptr, f.m, f.n,
+                                                    N, f.n, k, f.n, 0});
+                        }
+                        sb.copy(cs);
+                        sb.gemm(g);
+                    }
+                    for (int k0 = 0; k0 < (blk ? 0 : maxn); k0 += NB) {
                         std::vector<GemmDesc<T>> tr, up;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
@@ -432,6 +599,22 @@ struct Numeric : NumericBase {
                     if (!segs.empty())
                         steps.push_back(Step{S_GATHER, S.arena.add(segs), S.arena.add(rs),
                                              (int)segs.size(), rows, true});
+                    if (blk) {
+                        // FrontBlockLowerBackwardSolve (lower_solve/local_front_block.hpp:50-72):
+                        // Y := A21^T W (A21^H if Hermitian), X_s -= inv(ATL) Y
+                        std::vector<GemmDesc<T>> y, xs;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.u == 0 || f.n == 0) continue;
+                            y.push_back(GemmDesc<T>{L(s) + f.n, Wp(s), Tm(s), nullptr, nullptr, f.m,
+                                                    f.u, f.n, f.n, k, f.u, G_TRANSA | cA});
+                            xs.push_back(GemmDesc<T>{L(s), Tm(s), Xs(s), nullptr, nullptr, f.m, f.n,
+                                                     N, f.n, k, f.n, G_ACCUM | G_NEG});
+                        }
+                        sb.gemm(y);
+                        sb.gemm(xs);
+                        continue;
+                    }
                     sb.gemm(g);
                     const int maxq = (maxn + NB - 1) / NB;
                     for (int q = maxq - 1; q >= 0; --q) {
@@ -457,7 +640,7 @@ struct Numeric : NumericBase {
         }
         S.arena.finish();
         SolveSched& ref = S;
-        solves.emplace(k, std::move(sp));
+        solves.emplace(key, std::move(sp));
         return ref;
     }
 
@@ -484,12 +667,16 @@ struct Numeric : NumericBase {
                     break;
                 case S_GEMM:
                     launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                   s.variant, S ? 1 : 0, st);
+                                   s.variant, s.scaled ? 2 : (S ? 1 : 0), st);
                     break;
                 case S_GATHER:
                     launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
                                      S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k,
                                      P.dLow.as<int>(), st);
+                    break;
+                case S_COPY:
+                    launch_copy2d<T>(a.at<CopyDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                     s.nwork, st);
                     break;
                 case S_SEXTEND:
                     launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
@@ -509,9 +696,16 @@ struct Numeric : NumericBase {
 
     void factor(const V* vals) {
         hipStream_t st = P.stream;
+        if (block() && !blockBuilt) {
+            build_factor_schedule(barena, bsteps, true);
+            blockBuilt = true;
+        }
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
         launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate, st);
-        run(fsteps, farena, nullptr, 0);
+        if (block())
+            run(bsteps, barena, nullptr, 0);
+        else
+            run(fsteps, farena, nullptr, 0);
     }
 
     // x: V, N x k (ld ldx); nodal or original ordering; solved in place
@@ -529,7 +723,11 @@ struct Numeric : NumericBase {
         }
         launch_pull<T, V>(x, ldx, map, S.xn.template as<T>(), P.N, k, st);
         run(S.fwd, S.arena, &S, k);
-        launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k, P.dStatus.as<int>(), st);
+        // block fronts hold inv(ATL), so there is no separate diagonal solve
+        // (solve.hpp:48-53)
+        if (!block())
+            launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k,
+                                    P.dStatus.as<int>(), st);
         run(S.bwd, S.arena, &S, k);
         launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
         CQ_HIP(hipGetLastError());
@@ -552,7 +750,8 @@ struct Numeric : NumericBase {
             CQ_HIP(hipStreamSynchronize(P.stream));
             CQ_HIP(hipMemcpy(hL, tmp.p, mn * sizeof(V), hipMemcpyDeviceToHost));
         }
-        if (hdiag && f.n > 0) {
+        // block fronts keep no diagonal (ldl/local.hpp:106-107)
+        if (hdiag && f.n > 0 && !block()) {
             CQ_HIP(hipStreamSynchronize(P.stream));
             CQ_HIP(hipMemcpy(hdiag, D(s), f.n * sizeof(V), hipMemcpyDeviceToHost));
         }
@@ -739,7 +938,7 @@ static void dispatch(bool complex, int precision, F&& f) {
 
 void Plan::factor(const void* d_vals, int prec, int ftype) {
     if (prec != 0 && prec != 1) throw Error(1, "unknown precision");
-    if (ftype != 0) throw Error(1, "front type not supported by this build");
+    if (ftype != 0 && ftype != 1) throw Error(1, "unknown front type");
     if (!num || prec != precision) {
         num.reset();
         precision = prec;

@@ -667,7 +667,8 @@ public:
             detail::check(cliqg_symbolic_node(sym_.get(), s, in), "MirrorFronts");
             const int n = in[0], m = in[0] + in[5];
             localFronts[s].frontL.Resize(m, n);
-            localFronts[s].diag.Resize(n, 1);
+            // block fronts keep no diagonal (ldl/local.hpp:106-107)
+            localFronts[s].diag.Resize(BlockFactorization(frontType) ? 0 : n, 1);
             detail::check(cliqg_plan_get_front(plan_.get(), s, localFronts[s].frontL.Buffer(),
                                                BlockFactorization(frontType)
                                                    ? nullptr

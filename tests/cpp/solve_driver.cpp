@@ -2,7 +2,7 @@
 // operator, random X, Y := A X, natural nested dissection, LDL, Pull, Solve,
 // Push, error norms), written against the cliq:: surface of clique.hpp.
 // Exits non-zero if the relative error or the backward error is too large.
-//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [complex]
+//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [real|complex] [ldl|block]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -11,7 +11,7 @@
 using namespace cliq;
 
 template <typename F>
-int run(int n1, int n2, int n3, int numRhs, bool fp32) {
+int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
     mpi::Comm comm = mpi::COMM_WORLD;
     const int N = n1 * n2 * n3;
     DistSparseMatrix<F> A(N, comm);
@@ -49,7 +49,7 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32) {
     double lf, minf, maxf, gf;
     frontTree.FactorizationWork(lf, minf, maxf, gf);
     const double t0 = mpi::Time();
-    LDL(info, frontTree, LDL_1D, fp32 ? FACTOR_FP32 : FACTOR_FP64);
+    LDL(info, frontTree, ftype, fp32 ? FACTOR_FP32 : FACTOR_FP64);
     mpi::Barrier(comm);
     const double t1 = mpi::Time();
     std::printf("LDL: %.4f s, %.2f GFlop/s (device %.3f ms)\n", t1 - t0, gf / (1e9 * (t1 - t0)),
@@ -101,9 +101,11 @@ int main(int argc, char* argv[]) {
     const int k = argc > 4 ? std::atoi(argv[4]) : 5;
     const bool fp32 = argc > 5 && std::string(argv[5]) == "fp32";
     const bool cpx = argc > 6 && std::string(argv[6]) == "complex";
+    const SymmFrontType ftype = argc > 7 && std::string(argv[7]) == "block" ? BLOCK_LDL_1D : LDL_1D;
     int rc = 1;
     try {
-        rc = cpx ? run<std::complex<double>>(n1, n2, n3, k, fp32) : run<double>(n1, n2, n3, k, fp32);
+        rc = cpx ? run<std::complex<double>>(n1, n2, n3, k, fp32, ftype)
+                 : run<double>(n1, n2, n3, k, fp32, ftype);
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());
         rc = 2;

@@ -13,7 +13,9 @@ DRIVER = os.path.join(ROOT, "tests", "cpp", "solve_driver")
 @pytest.mark.parametrize("args", [["30", "30", "30", "5"],               # config 1 shape
                                   ["17", "9", "13", "3", "fp32"],
                                   ["12", "12", "12", "2", "fp64", "complex"],
-                                  ["12", "12", "12", "2", "fp32", "complex"]])
+                                  ["12", "12", "12", "2", "fp32", "complex"],
+                                  ["16", "14", "12", "3", "fp64", "real", "block"],
+                                  ["12", "12", "12", "2", "fp32", "complex", "block"]])
 def test_cpp_driver(device, args):
     if not os.path.exists(DRIVER):
         pytest.fail("tests/cpp/solve_driver is not built (run __graft_entry__.build())")

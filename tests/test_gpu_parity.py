@@ -212,3 +212,97 @@ def test_batched_2d_problems_one_forest(device, precision):
         eta = backward_error(m.indptr, m.indices, m.data, xg[o:o + n], b[o:o + n])
         assert eta.max() <= 1e-10
         o += n
+
+
+BLOCK_CASES = [
+    ("box", (7, 5, 3), 4, False),
+    ("box", (17, 1, 1), 2, False),
+    ("lap", 11, 16, False),
+    ("lap", 20, 128, False),    # root n = 400: several 64-wide blocks, a partial last one
+    ("helm", 8, 16, True),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,cut,cplx", BLOCK_CASES)
+def test_block_ldl_fronts_and_solve_match_oracle(device, kind, n, cut, cplx):
+    """BLOCK_LDL (ldl/local_front_block.hpp:25-73): ATL holds the full
+    symmetric inverse of L11 D L11^T, ABL the assembled A21; the block solves
+    (lower_solve/local_front_block.hpp) are products with those blocks."""
+    ip, ind, v, dims = problem(kind, n)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut, is_complex=cplx, front_type="block_ldl")
+    t, f = oracle_tree(ip, ind, v, dims, cut, is_complex=cplx, block=True)
+    worst = 0.0
+    for s in range(sym.num_nodes):
+        Lg, dg = ft.front(s)
+        Lo, _ = f.front(s)
+        assert dg.size == 0
+        scale = max(np.abs(Lo).max(initial=0), 1.0)
+        worst = max(worst, np.abs(Lg - Lo).max(initial=0) / scale)
+        nn = Lg.shape[1]
+        atl = Lg[:nn]
+        assert np.array_equal(atl, atl.T)    # MakeSymmetric: both triangles written
+    assert worst <= 1e-11, worst
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    assert rel_diff(xg, f.solve(b)) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_block_ldl_hermitian(device):
+    ip, ind, v, dims = hermitian_matrix(6)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 16, is_complex=True, conjugate=True,
+                       front_type="block_ldl")
+    t, f = oracle_tree(ip, ind, v, dims, 16, is_complex=True, conjugate=True, block=True)
+    for s in range(sym.num_nodes):
+        Lg, _ = ft.front(s)
+        Lo, _ = f.front(s)
+        nn = Lg.shape[1]
+        assert np.abs(Lg - Lo).max(initial=0) <= 1e-11 * max(np.abs(Lo).max(initial=0), 1)
+        atl = Lg[:nn]
+        assert np.array_equal(atl, atl.conj().T)   # Hermitian inverse, real diagonal
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal((N, 2)) + 1j * rng.standard_normal((N, 2))
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    assert rel_diff(xg, f.solve(b)) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,cut,cplx", [("lap", 30, 128, False), ("pml", 16, 64, True)])
+def test_block_ldl_fp32_with_refinement(device, kind, n, cut, cplx):
+    ip, ind, v, dims = problem(kind, n)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut, is_complex=cplx, precision="f32",
+                       front_type="block_ldl")
+    rng = np.random.default_rng(13)
+    b = rng.standard_normal((N, 4)) + (1j * rng.standard_normal((N, 4)) if cplx else 0)
+    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=30)
+    eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
+    assert info["converged"] and eta.max() <= 1e-10, (info, eta)
+
+
+@pytest.mark.gpu
+def test_refactor_switches_between_ldl_and_block(device):
+    """One plan refactored as LDL, BLOCK_LDL, then LDL again gives the same solution."""
+    ip, ind, v, dims = problem("lap", 12)
+    N = ip.size - 1
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=32)
+    ft = cq.DistSymmFrontTree(sym, ip, ind)
+    b = np.random.default_rng(14).standard_normal((N, 2))
+    xs = []
+    for ftype in ("ldl", "block_ldl", "ldl"):
+        ft.factor(torch.tensor(v, device="cuda:0"), front_type=ftype)
+        x = to_dev(b)
+        ft.solve(x)
+        xs.append(x.cpu().numpy())
+    assert rel_diff(xs[1], xs[0]) <= 1e-12
+    assert np.array_equal(xs[2], xs[0])
