@@ -41,6 +41,41 @@ __device__ __forceinline__ int find_segment(const int* __restrict__ start, int n
     return lo;
 }
 
+// Load through the global address space. Operand pointers come from
+// descriptor arrays, so the compiler cannot prove where they point and would
+// emit flat loads, which also count against the LDS wait counter: every LDS
+// wait in the GEMM's MFMA loop would then wait for the prefetch too.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float gload(const float* p) {
+    return *(const __attribute__((address_space(1))) float*)(p);
+}
+__device__ __forceinline__ double gload(const double* p) {
+    return *(const __attribute__((address_space(1))) double*)(p);
+}
+// complex entries are loaded as two-element vectors (a struct copy through
+// a global pointer would still become a flat load)
+__device__ __forceinline__ cfloat gload(const cfloat* p) {
+    const f2v v = *(const __attribute__((address_space(1))) f2v*)(p);
+    return cfloat{v.x, v.y};
+}
+__device__ __forceinline__ cdouble gload(const cdouble* p) {
+    const d2v v = *(const __attribute__((address_space(1))) d2v*)(p);
+    return cdouble{v.x, v.y};
+}
+__device__ __forceinline__ void gstore(float* p, float v) {
+    *(__attribute__((address_space(1))) float*)(p) = v;
+}
+__device__ __forceinline__ void gstore(double* p, double v) {
+    *(__attribute__((address_space(1))) double*)(p) = v;
+}
+__device__ __forceinline__ void gstore(cfloat* p, cfloat v) {
+    *(__attribute__((address_space(1))) f2v*)(p) = f2v{v.x, v.y};
+}
+__device__ __forceinline__ void gstore(cdouble* p, cdouble v) {
+    *(__attribute__((address_space(1))) d2v*)(p) = d2v{v.x, v.y};
+}
+
 __device__ __forceinline__ void record_status(int* status, int code, int front, int col) {
     if (atomicCAS(status, 0, code) == 0) {
         status[1] = front;
@@ -194,7 +229,6 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
     const int ldsB = kB * (TN + LPAD) + jB, ldsBStep = dKB * (TN + LPAD) + dJB;
 
     T ra[LA], rb[LB];
-    T rd = one<T>();
     const T* pA = nullptr;
     const T* pB = nullptr;
     auto start_at = [&](int k0) {
@@ -203,32 +237,32 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
         pB = transB ? d.B + (col0 + jB) + (int64_t)(k0 + kB) * d.ldb
                     : d.B + (k0 + kB) + (int64_t)(col0 + jB) * d.ldb;
     };
+    // Loads never branch: an element outside the matrix reads the operand's
+    // first entry and is replaced by zero when it is stored to LDS, so all of
+    // a chunk's loads are in flight together and the only wait is before the
+    // store (a divergent load would wait inside its branch).
+    W rdw = one<W>();
+    unsigned okA = 0, okB = 0;
+    bool okD = false;
     auto load = [&](int k0) {
-        if constexpr (SCALED)
-            if (tid < TK) {
-                rd = zero<T>();
-                if (k0 + tid < d.K)
-                    rd = (d.flags & G_INVK) ? cvt<T>(one<W>() / scaleK[k0 + tid])
-                                            : cvt<T>(scaleK[k0 + tid]);
-            }
         const bool fullK = k0 + TK <= d.K;
+        if constexpr (SCALED) {
+            okD = tid < TK && k0 + tid < d.K;
+            rdw = gload(scaleK + (okD ? k0 + tid : 0));
+        }
+        okA = 0;
+        okB = 0;
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
-            T v = zero<T>();
-            if (((rowOK >> r) & 1u) && (fullK || k0 + kA + r * dKA < d.K)) {
-                v = pA[r * offA];
-                if (conjA) v = conj_(v);
-            }
-            ra[r] = v;
+            const bool ok = ((rowOK >> r) & 1u) && (fullK || k0 + kA + r * dKA < d.K);
+            okA |= ok ? 1u << r : 0u;
+            ra[r] = gload(ok ? pA + r * offA : d.A);
         }
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
-            T v = zero<T>();
-            if (((colOK >> r) & 1u) && (fullK || k0 + kB + r * dKB < d.K)) {
-                v = pB[r * offB];
-                if (conjB) v = conj_(v);
-            }
-            rb[r] = v;
+            const bool ok = ((colOK >> r) & 1u) && (fullK || k0 + kB + r * dKB < d.K);
+            okB |= ok ? 1u << r : 0u;
+            rb[r] = gload(ok ? pB + r * offB : d.B);
         }
         pA += stepA;
         pB += stepB;
@@ -236,8 +270,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
     auto store = [&](int buf) {
         if constexpr (SCALED)
             if (tid < TK) {
-                As[buf][0][tid][TM] = re(rd);
-                if constexpr (CPX) As[buf][NP - 1][tid][TM] = im(rd);
+                T dv = zero<T>();
+                if (okD) dv = (d.flags & G_INVK) ? cvt<T>(one<W>() / rdw) : cvt<T>(rdw);
+                As[buf][0][tid][TM] = re(dv);
+                if constexpr (CPX) As[buf][NP - 1][tid][TM] = im(dv);
             }
         R* a0 = &As[buf][0][0][0];
         R* a1 = &As[buf][NP - 1][0][0];
@@ -245,13 +281,17 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
         R* b1 = &Bs[buf][NP - 1][0][0];
 #pragma unroll
         for (int r = 0; r < LA; ++r) {
-            a0[ldsA + r * ldsAStep] = re(ra[r]);
-            if constexpr (CPX) a1[ldsA + r * ldsAStep] = im(ra[r]);
+            T v = ((okA >> r) & 1u) ? ra[r] : zero<T>();
+            if (conjA) v = conj_(v);
+            a0[ldsA + r * ldsAStep] = re(v);
+            if constexpr (CPX) a1[ldsA + r * ldsAStep] = im(v);
         }
 #pragma unroll
         for (int r = 0; r < LB; ++r) {
-            b0[ldsB + r * ldsBStep] = re(rb[r]);
-            if constexpr (CPX) b1[ldsB + r * ldsBStep] = im(rb[r]);
+            T v = ((okB >> r) & 1u) ? rb[r] : zero<T>();
+            if (conjB) v = conj_(v);
+            b0[ldsB + r * ldsBStep] = re(v);
+            if constexpr (CPX) b1[ldsB + r * ldsBStep] = im(v);
         }
     };
 
@@ -332,13 +372,13 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
                 if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
                 if (neg) v = -v;
                 T* p = d.C + gi + (int64_t)gj * d.ldc;
-                if (accum) v = *p + v;
+                if (accum) v = gload(p) + v;
                 if (symm) {
                     // Hermitian output (conj mirror) has a real diagonal
                     if (conjA && gi == gj) v = make_t<T>(re(v), R(0));
-                    if (gi > gj) d.C[gj + (int64_t)gi * d.ldc] = conjA ? conj_(v) : v;
+                    if (gi > gj) gstore(d.C + gj + (int64_t)gi * d.ldc, conjA ? conj_(v) : v);
                 }
-                *p = v;
+                gstore(p, v);
             }
 }
 
