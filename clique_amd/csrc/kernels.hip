@@ -76,6 +76,18 @@ __device__ __forceinline__ void gstore(cdouble* p, cdouble v) {
     *(__attribute__((address_space(1))) d2v*)(p) = d2v{v.x, v.y};
 }
 
+// Select by value, component by component: a conditional between two
+// lvalues of a struct type selects their addresses and sends the operands to
+// scratch memory.
+__device__ __forceinline__ float pick(bool c, float a, float b) { return c ? a : b; }
+__device__ __forceinline__ double pick(bool c, double a, double b) { return c ? a : b; }
+__device__ __forceinline__ cfloat pick(bool c, cfloat a, cfloat b) {
+    return cfloat{c ? a.x : b.x, c ? a.y : b.y};
+}
+__device__ __forceinline__ cdouble pick(bool c, cdouble a, cdouble b) {
+    return cdouble{c ? a.x : b.x, c ? a.y : b.y};
+}
+
 __device__ __forceinline__ void record_status(int* status, int code, int front, int col) {
     if (atomicCAS(status, 0, code) == 0) {
         status[1] = front;
@@ -529,7 +541,9 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 //     li(q,i) -= (a(q,j) / d_j) li(j,i)           (upper, i <= j < q; li(j,j) = 1)
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
+// (two waves per SIMD are plenty for a latency-bound kernel; the complex
+// instantiations need the registers: at the default budget they spill)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
                                                   bool herm, int* status) {
     using W = typename Scalar<T>::wide;
     __shared__ W col[2][NB];
@@ -540,12 +554,23 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
     const int b = d.b, tid = threadIdx.x;
     const int ti = tid >> 4, tj = tid & 15;
     W M[4][4];
+    // all sixteen loads in flight at once (an element outside the lower
+    // triangle reads A11(0,0) and is replaced by zero)
+    T raw[4][4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = 4 * ti + r, c = 4 * tj + q;
-            M[r][q] = (i < b && c < b && i >= c) ? cvt<W>(d.A11[i + (int64_t)c * d.lda]) : W{};
+            const bool in = i < b && c < b && i >= c;
+            raw[r][q] = gload(d.A11 + (in ? i + (int64_t)c * d.lda : 0));
+        }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * ti + r, c = 4 * tj + q;
+            M[r][q] = (i < b && c < b && i >= c) ? cvt<W>(raw[r][q]) : W{};
         }
     W* Dout = reinterpret_cast<W*>(d.D);
     for (int j = 0; j < b; ++j) {
@@ -568,23 +593,26 @@ __global__ __launch_bounds__(256) void k_diag_ldl(const DiagDesc<T>* __restrict_
             dval[j] = piv;
             dinv[j] = ip;
         }
-        W ci[4], cc[4];
+        W x[4], cc[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            ci[r] = col[buf][4 * ti + r];
+            x[r] = col[buf][4 * ti + r] * ip;
             cc[r] = col[buf][4 * tj + r];
         }
+        // Branch-free update: lower (i >= c > j): M -= (a_ij / d_j) op(a_cj);
+        // upper (i <= j < c): M -= a_cj / d_j * l_ij with l_jj = 1. Every
+        // thread evaluates one product per element and keeps it by select
+        // (divergent branches per element cost far more than the product).
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = 4 * ti + r, c = 4 * tj + q;
-                if (i >= c) {
-                    if (c > j) M[r][q] = M[r][q] - ci[r] * ip * cj(cc[q], herm);
-                } else if (i <= j && c > j) {
-                    const W lij = (i == j) ? one<W>() : ci[r];
-                    M[r][q] = M[r][q] - cc[q] * ip * lij;
-                }
+                const bool lowerEl = i >= c;
+                const bool act = c > j && (lowerEl || i <= j);
+                const W f = pick(!lowerEl && i == j, ip, x[r]);
+                const W g = pick(lowerEl && herm, conj_(cc[q]), cc[q]);
+                M[r][q] = pick(act, M[r][q] - f * g, M[r][q]);
             }
     }
     __syncthreads();

@@ -14,6 +14,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -238,11 +239,14 @@ struct StepBuilder {
 // ---------------------------------------------------------------------------
 // Numeric state for one working type T (float, double, cfloat, cdouble)
 // ---------------------------------------------------------------------------
+// Outer panel width of the front factorization (a multiple of NB).
+constexpr int OB = 256;
+
 template <class T>
 struct Numeric : NumericBase {
     using V = typename Scalar<T>::wide;
     Plan& P;
-    DevBuf fpool, upool, dpool, linv;
+    DevBuf fpool, upool, dpool, linv, linvw, wtmp;
     Arena farena, barena;
     std::vector<Step> fsteps, bsteps;   // LDL and block-LDL factor schedules
     bool blockBuilt = false;
@@ -264,6 +268,17 @@ struct Numeric : NumericBase {
         upool.alloc(std::max<int64_t>(P.updateEntries, 1) * sizeof(T));
         dpool.alloc(std::max(P.N, 1) * sizeof(V));
         linv.alloc(std::max<int64_t>(P.linvEntries, 1) * sizeof(T));
+        linvw.alloc(std::max<int64_t>(P.linvWEntries, 1) * sizeof(T));
+        {
+            // n x 64 panel products of the wide inverses, reused level by level
+            int64_t mx = 0;
+            for (const auto& lv : P.levels) {
+                int64_t a = 0;
+                for (int s : lv) a += (int64_t)P.fronts[s].n * NB;
+                mx = std::max(mx, a);
+            }
+            wtmp.alloc(std::max<int64_t>(mx, 1) * sizeof(T));
+        }
         build_factor_schedule(farena, fsteps, false);
     }
 
@@ -272,7 +287,7 @@ struct Numeric : NumericBase {
     size_t device_bytes() const override {
         size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
                    barena.dev.bytes + bsave.bytes + bsinv.bytes + btmp.bytes + rbuf.bytes +
-                   identity.bytes;
+                   identity.bytes + linvw.bytes + wtmp.bytes;
         for (auto& kv : solves)
             b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
                  kv.second->tmp.bytes;
@@ -283,6 +298,11 @@ struct Numeric : NumericBase {
     T* U(int s) const { return upool.as<T>() + P.fronts[s].U; }
     V* D(int s) const { return dpool.as<V>() + P.fronts[s].off; }
     T* Li(int s, int k0) const { return linv.as<T>() + P.fronts[s].Linv + (int64_t)(k0 / NB) * NB * NB; }
+    // wide inverse block starting at column w0 (a multiple of SB), ld min(SB, n - w0)
+    T* LW(int s, int w0) const {
+        const int SB = P.sb;
+        return linvw.as<T>() + P.fronts[s].LinvW + (int64_t)(w0 / SB) * SB * SB;
+    }
 
     // Factor schedule. Block LDL (ldl/local_front_block.hpp:25-73) adds, per
     // level: save A21 after the extend-add; after the Schur update, the
@@ -366,37 +386,53 @@ struct Numeric : NumericBase {
             }
             int maxn = 0;
             for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
-            for (int k = 0; k < maxn; k += NB) {
-                std::vector<DiagDesc<T>> dd;
-                std::vector<GemmDesc<T>> trsm, upd;
+            // Two-level panels: 64-wide diagonal blocks whose updates stay
+            // inside an OB-wide outer panel, then one rank-OB update of the
+            // front's trailing columns (K = OB instead of K = 64 passes).
+            for (int k0 = 0; k0 < maxn; k0 += OB) {
+                for (int k = k0; k < std::min(maxn, k0 + OB); k += NB) {
+                    std::vector<DiagDesc<T>> dd;
+                    std::vector<GemmDesc<T>> trsm, upd;
+                    for (int s : lv) {
+                        const FrontLayout& f = P.fronts[s];
+                        if (f.n <= k) continue;
+                        const int b = std::min(NB, f.n - k);
+                        const int c1 = std::min(f.n, k0 + OB);   // end of the outer panel
+                        T* base = L(s);
+                        dd.push_back(DiagDesc<T>{base + k + (int64_t)k * f.m, D(s) + k, Li(s, k), f.m,
+                                                 b, s, k});
+                        T* a21 = base + (k + b) + (int64_t)k * f.m;
+                        if (f.m - k - b > 0)
+                            trsm.push_back(GemmDesc<T>{a21, Li(s, k), a21, nullptr, D(s) + k, f.m, NB,
+                                                       f.m, f.m - k - b, b, b, G_TRANSB | herm});
+                        if (c1 - k - b > 0) {
+                            T* a22 = base + (k + b) + (int64_t)(k + b) * f.m;
+                            upd.push_back(GemmDesc<T>{a21, a21, a22, D(s) + k, nullptr, f.m, f.m, f.m,
+                                                      f.m - k - b, c1 - k - b, b,
+                                                      G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                        }
+                    }
+                    if (!dd.empty()) {
+                        double fl = 0.0;
+                        for (const auto& x : dd) fl += (double)x.b * x.b * x.b / 3.0;
+                        steps.push_back(Step{S_DIAG, arena.add(dd), 0, (int)dd.size(), 0, false,
+                                             fl * (Scalar<T>::complex ? 4 : 1), 0.0});
+                    }
+                    sb.gemm(trsm, true);
+                    sb.gemm(upd);
+                }
+                std::vector<GemmDesc<T>> outer;
                 for (int s : lv) {
                     const FrontLayout& f = P.fronts[s];
-                    if (f.n <= k) continue;
-                    const int b = std::min(NB, f.n - k);
-                    T* base = L(s);
-                    dd.push_back(DiagDesc<T>{base + k + (int64_t)k * f.m, D(s) + k, Li(s, k), f.m, b,
-                                             s, k});
-                    if (f.m - k - b > 0) {
-                        T* a21 = base + (k + b) + (int64_t)k * f.m;
-                        trsm.push_back(GemmDesc<T>{a21, Li(s, k), a21, nullptr, D(s) + k, f.m, NB,
-                                                   f.m, f.m - k - b, b, b, G_TRANSB | herm});
-                    }
-                    if (f.n - k - b > 0) {
-                        T* a21 = base + (k + b) + (int64_t)k * f.m;
-                        T* a22 = base + (k + b) + (int64_t)(k + b) * f.m;
-                        upd.push_back(GemmDesc<T>{a21, a21, a22, D(s) + k, nullptr, f.m, f.m, f.m,
-                                                  f.m - k - b, f.n - k - b, b,
-                                                  G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
-                    }
+                    const int c1 = std::min(f.n, k0 + OB);
+                    if (f.n <= k0 || f.n - c1 <= 0) continue;
+                    T* l = L(s) + c1 + (int64_t)k0 * f.m;
+                    T* a22 = L(s) + c1 + (int64_t)c1 * f.m;
+                    outer.push_back(GemmDesc<T>{l, l, a22, D(s) + k0, nullptr, f.m, f.m, f.m, f.m - c1,
+                                                f.n - c1, c1 - k0,
+                                                G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
                 }
-                if (!dd.empty()) {
-                    double fl = 0.0;
-                    for (const auto& x : dd) fl += (double)x.b * x.b * x.b / 3.0;
-                    steps.push_back(Step{S_DIAG, arena.add(dd), 0, (int)dd.size(), 0, false,
-                                          fl * (Scalar<T>::complex ? 4 : 1), 0.0});
-                }
-                sb.gemm(trsm, true);
-                sb.gemm(upd);
+                sb.gemm(outer);
             }
             std::vector<GemmDesc<T>> syrk;
             for (int s : lv) {
@@ -408,8 +444,59 @@ struct Numeric : NumericBase {
             }
             sb.gemm(syrk);
             if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp);
+            else wide_inverse(sb, lv);
         }
         arena.finish();
+    }
+
+    // Inverses of the SB-wide diagonal blocks of each L11 (used by the LDL
+    // solves), from the 64-wide ones: inside a wide block, sub-blocks from the
+    // last, X_jj = Linv_jj and X21_j = -(X22 L21_j) Linv_jj.
+    void wide_inverse(StepBuilder<T>& sb, const std::vector<int>& lv) {
+        const int SB = P.sb;
+        struct WB { int s, w0, sbw; int64_t tmp; };
+        std::vector<WB> wbs;
+        std::vector<ZeroSeg<T>> zs;
+        int64_t tmpTotal = 0;
+        for (int s : lv) {
+            const FrontLayout& f = P.fronts[s];
+            int64_t entries = 0;
+            for (int w0 = 0; w0 < f.n; w0 += SB) {
+                const int sbw = std::min(SB, f.n - w0);
+                wbs.push_back(WB{s, w0, sbw, tmpTotal});
+                tmpTotal += (int64_t)sbw * NB;
+                entries += (int64_t)sbw * sbw;
+            }
+            if (entries) zs.push_back(ZeroSeg<T>{LW(s, 0), entries});
+        }
+        if (tmpTotal * (int64_t)sizeof(T) > (int64_t)wtmp.bytes)
+            throw Error(5, "wide inverse workspace smaller than a level needs");
+        sb.zero(zs);
+        for (int t = 0; t < SB / NB; ++t) {
+            std::vector<CopyDesc<T>> diagBlocks;
+            std::vector<GemmDesc<T>> trmm, scal;
+            for (const WB& w : wbs) {
+                const FrontLayout& f = P.fronts[w.s];
+                const int q = (w.sbw + NB - 1) / NB;
+                if (q <= t) continue;
+                const int lo = (q - 1 - t) * NB, j0 = w.w0 + lo;
+                const int b = std::min(NB, w.sbw - lo), r = w.sbw - lo - b;
+                const int64_t ld = w.sbw;
+                T* X = LW(w.s, w.w0);
+                diagBlocks.push_back(CopyDesc<T>{Li(w.s, j0), X + lo + lo * ld, NB, ld, b, b});
+                if (r > 0) {
+                    T* tp = wtmp.as<T>() + w.tmp;
+                    trmm.push_back(GemmDesc<T>{X + (lo + b) + (lo + b) * ld,
+                                               L(w.s) + (j0 + b) + (int64_t)j0 * f.m, tp, nullptr,
+                                               nullptr, ld, f.m, ld, r, b, r, G_KHI});
+                    scal.push_back(GemmDesc<T>{tp, Li(w.s, j0), X + (lo + b) + lo * ld, nullptr,
+                                               nullptr, ld, NB, ld, r, b, b, G_NEG});
+                }
+            }
+            sb.copy(diagBlocks);
+            sb.gemm(trmm, true);
+            sb.gemm(scal, true);
+        }
     }
 
     template <class FS, class FX, class FT>
@@ -470,6 +557,7 @@ struct Numeric : NumericBase {
     // X(0:k) -= L(k:k+b, 0:k)^T X_k.
     SolveSched& schedule(int k) {
         const bool blk = block();
+        const int SB = P.sb;
         const int key = 2 * k + (blk ? 1 : 0);
         auto it = solves.find(key);
         if (it != solves.end()) return *it->second;
@@ -477,14 +565,16 @@ struct Numeric : NumericBase {
         SolveSched& S = *sp;
         S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
         S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
+        // per-front k-column workspace: n rows for block fronts, one wide
+        // panel (at most SB rows) for LDL fronts
         std::vector<int64_t> tmpOff(P.numFronts, 0);
-        if (blk) {
+        {
             int64_t mx = 0;
             for (int h = 0; h < P.numLevels; ++h) {
                 int64_t a = 0;
                 for (int s : P.levels[h]) {
                     tmpOff[s] = a;
-                    a += (int64_t)P.fronts[s].n * k;
+                    a += (int64_t)(blk ? P.fronts[s].n : std::min(P.fronts[s].n, SB)) * k;
                 }
                 mx = std::max(mx, a);
             }
@@ -553,23 +643,28 @@ struct Numeric : NumericBase {
                         sb.copy(cs);
                         sb.gemm(g);
                     }
-                    for (int k0 = 0; k0 < (blk ? 0 : maxn); k0 += NB) {
+                    // SB-wide panels: Y := inv(L_pp) X_p (wide inverse, out of
+                    // place), X(below) -= L(below, p) Y, X_p := Y
+                    for (int w0 = 0; w0 < (blk ? 0 : maxn); w0 += SB) {
                         std::vector<GemmDesc<T>> tr, up;
+                        std::vector<CopyDesc<T>> cp;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
-                            if (f.n <= k0) continue;
-                            const int b = std::min(NB, f.n - k0);
-                            T* x = Xs(s) + k0;
-                            tr.push_back(GemmDesc<T>{Li(s, k0), x, x, nullptr, nullptr, NB, N, N, b,
-                                                     k, b, 0});
-                            const T* lcol = L(s) + (int64_t)k0 * f.m;
-                            if (f.n - k0 - b > 0)
-                                up.push_back(GemmDesc<T>{lcol + k0 + b, x, x + b, nullptr, nullptr,
-                                                         f.m, N, N, f.n - k0 - b, k, b,
-                                                         G_ACCUM | G_NEG});
+                            if (f.n <= w0) continue;
+                            const int sbw = std::min(SB, f.n - w0);
+                            T* x = Xs(s) + w0;
+                            T* y = Tm(s);
+                            tr.push_back(GemmDesc<T>{LW(s, w0), x, y, nullptr, nullptr, sbw, N, sbw,
+                                                     sbw, k, sbw, G_KHI});
+                            if (f.n - w0 - sbw > 0)
+                                up.push_back(GemmDesc<T>{L(s) + (w0 + sbw) + (int64_t)w0 * f.m, y,
+                                                         x + sbw, nullptr, nullptr, f.m, sbw, N,
+                                                         f.n - w0 - sbw, k, sbw, G_ACCUM | G_NEG});
+                            cp.push_back(CopyDesc<T>{y, x, sbw, N, sbw, k});
                         }
                         sb.gemm(tr, true);
                         sb.gemm(up);
+                        sb.copy(cp);
                     }
                     // the front's update rows in one product with K = n
                     std::vector<GemmDesc<T>> g;
@@ -616,24 +711,30 @@ struct Numeric : NumericBase {
                         continue;
                     }
                     sb.gemm(g);
-                    const int maxq = (maxn + NB - 1) / NB;
-                    for (int q = maxq - 1; q >= 0; --q) {
-                        const int k0 = q * NB;
+                    // SB-wide panels from the last: Y := inv(L_pp)^T X_p (^H if
+                    // Hermitian), X(0:w0) -= L(p, 0:w0)^T Y, X_p := Y
+                    const int maxw = (maxn + SB - 1) / SB;
+                    for (int pw = maxw - 1; pw >= 0; --pw) {
+                        const int w0 = pw * SB;
                         std::vector<GemmDesc<T>> tr, up;
+                        std::vector<CopyDesc<T>> cp;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
-                            if (f.n <= k0) continue;
-                            const int b = std::min(NB, f.n - k0);
-                            T* x = Xs(s) + k0;
-                            tr.push_back(GemmDesc<T>{Li(s, k0), x, x, nullptr, nullptr, NB, N, N, b,
-                                                     k, b, G_TRANSA | cA});
-                            if (k0 > 0)
-                                up.push_back(GemmDesc<T>{L(s) + k0, x, Xs(s), nullptr, nullptr, f.m,
-                                                         N, N, k0, k, b,
+                            if (f.n <= w0) continue;
+                            const int sbw = std::min(SB, f.n - w0);
+                            T* x = Xs(s) + w0;
+                            T* y = Tm(s);
+                            tr.push_back(GemmDesc<T>{LW(s, w0), x, y, nullptr, nullptr, sbw, N, sbw,
+                                                     sbw, k, sbw, G_TRANSA | G_KLO | cA});
+                            if (w0 > 0)
+                                up.push_back(GemmDesc<T>{L(s) + w0, y, Xs(s), nullptr, nullptr, f.m,
+                                                         sbw, N, w0, k, sbw,
                                                          G_TRANSA | G_ACCUM | G_NEG | cA});
+                            cp.push_back(CopyDesc<T>{y, x, sbw, N, sbw, k});
                         }
                         sb.gemm(tr, true);
                         sb.gemm(up);
+                        sb.copy(cp);
                     }
                 }
             }
@@ -766,6 +867,11 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     : sym(std::move(symIn)), complex(cpx), conjugate(conj), stream(st) {
     const Symbolic& S = *sym;
     N = S.N;
+    if (const char* e = std::getenv("CLIQG_SOLVE_BLOCK")) {
+        const int v = std::atoi(e);
+        if (v < NB || v % NB) throw Error(1, "CLIQG_SOLVE_BLOCK must be a positive multiple of 64");
+        sb = v;
+    }
     numFronts = (int)S.nodes.size();
     nnz = rowOff[N] - rowOff[0];
     hostRowOff.assign(rowOff, rowOff + N + 1);
@@ -773,7 +879,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     const int* colp = cols + rowOff[0];
 
     fronts.resize(numFronts);
-    int64_t Lp = 0, Up = 0, Wp = 0, Ip = 0;
+    int64_t Lp = 0, Up = 0, Wp = 0, Ip = 0, IWp = 0;
     std::vector<int> relAll, lowAll;
     int maxh = 0;
     for (int s = 0; s < numFronts; ++s) {
@@ -795,6 +901,11 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
         f.U = 0;
         f.W = Wp; Wp += f.u;
         f.Linv = Ip; Ip += (int64_t)((f.n + NB - 1) / NB) * NB * NB;
+        f.LinvW = IWp;
+        for (int j0 = 0; j0 < f.n; j0 += sb) {
+            const int64_t w = std::min(sb, f.n - j0);
+            IWp += w * w;
+        }
         f.low = (int64_t)lowAll.size();
         lowAll.insert(lowAll.end(), nd.lowerStruct.begin(), nd.lowerStruct.end());
     }
@@ -814,6 +925,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     factorEntries = Lp;
     wRows = Wp;
     linvEntries = Ip;
+    linvWEntries = IWp;
     numLevels = numFronts ? maxh + 1 : 0;
     levels.assign(numLevels, {});
     for (int s = 0; s < numFronts; ++s) levels[fronts[s].height].push_back(s);

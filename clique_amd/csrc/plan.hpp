@@ -40,7 +40,12 @@ struct FrontLayout {
     int64_t low;    // offset of lowerStruct (length u)
     int64_t W;      // solve workspace row offset (prefix of u)
     int64_t Linv;   // inverse diagonal block pool offset
+    int64_t LinvW;  // wide inverse diagonal block pool offset (blocks of SB)
 };
+
+// Default width of the diagonal blocks whose inverses the LDL solves apply
+// (a multiple of the 64-wide factor panels).
+constexpr int SB_DEFAULT = 128;
 
 // Opaque per-working-type numeric state.
 struct NumericBase {
@@ -73,7 +78,8 @@ public:
     int64_t nnz = 0;
     std::vector<FrontLayout> fronts;
     std::vector<std::vector<int>> levels;   // fronts of each height
-    int64_t factorEntries = 0, updateEntries = 0, linvEntries = 0, wRows = 0;
+    int64_t factorEntries = 0, updateEntries = 0, linvEntries = 0, linvWEntries = 0, wRows = 0;
+    int sb = SB_DEFAULT;   // solve block width (CLIQG_SOLVE_BLOCK overrides; multiple of 64)
     std::vector<int64_t> hostRowOff;
     hipStream_t stream = nullptr;
     bool ownStream = false;
