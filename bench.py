@@ -118,6 +118,39 @@ def cpu_baseline(budget_s=20.0):
                        f"Helmholtz operator: {flops / 1e9:.1f} GFLOP in {dt:.2f} s")}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gemm_update_pml120_fp32_traffic.json")
+
+
+def measured_traffic(cfg, n):
+    """HBM bytes per launch of the factor's update GEMM, from the committed
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE over `bench.py
+    --factor-only`, each scaled by the same counter on a copy of known size);
+    None for other workloads or when no measurement is committed."""
+    if cfg != 5 or n not in (None, 120) or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as fh:
+        t = json.load(fh)
+    return t["bytes_per_launch"], os.path.relpath(TRAFFIC_FILE, ROOT)
+
+
+def factor_only(args):
+    """Two FP32 factorizations and a 1 GiB device copy, nothing else: the
+    program the byte counters are collected on (the copy calibrates them)."""
+    import torch
+    import clique_amd as cq
+    ip, ind, v, dims, b, cplx, desc = workload(args.config, args.n)
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
+    vals = torch.tensor(v, device="cuda:0")
+    p = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+    for _ in range(2):
+        p.factor(vals, precision="f32")
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"factor_ms": p.last_factor_ms(), "copy_bytes": src.numel() * 4}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,11 +164,16 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-block", action="store_true",
                     help="skip the BLOCK_LDL (selective inversion) FP32+IR line")
+    ap.add_argument("--factor-only", action="store_true",
+                    help="two FP32 factorizations and a calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.factor_only:
+        factor_only(args)
+        return
 
     import torch
     import clique_amd as cq
@@ -268,13 +306,15 @@ def main():
                 "factor_tflops": flops / (np.mean(f64_ms) * 1e-3) / 1e12,
                 "refine_iterations": infos64[-1]["iterations"],
                 "backward_error": infos64[-1]["backward_error"],
-                "gemm_tflops": prof64["gemm"]["flops"] / (prof64["gemm"]["ms"] * 1e-3) / 1e12}
+                "gemm_update_tflops": prof64["gemm_update"]["flops"]
+                / (prof64["gemm_update"]["ms"] * 1e-3) / 1e12}
         print(f"# fp64: {t64 * 1e3:.1f} ms/step, factor {np.mean(f64_ms):.1f} ms",
               file=sys.stderr, flush=True)
         del p64
 
-    g = prof32["gemm"]
+    g = prof32["gemm_update"]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
+    traffic, traffic_src = measured_traffic(args.config, args.n)
     def families(prof):
         return {name: {"launches": e["launches"], "ms": round(e["ms"], 3),
                        "tflops": (e["flops"] / (e["ms"] * 1e-3) / 1e12) if e["flops"] else None,
@@ -304,9 +344,11 @@ def main():
                     "device_bytes": mem32},
         "fp64": fp64,
         "speedup_fp32ir_vs_fp64": (fp64["time_to_tol_ms"] / (t32 * 1e3)) if fp64 else None,
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (grouped MFMA GEMM, FP32 factor)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "k_gemm<cfloat,64,64,2> (pivot-scaled Schur/panel updates, FP32 factor)",
                      "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": None,
+                     "frac": achieved / PEAK["fp32"], "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / g["launches"],
                      "flops_per_launch": g["flops"] / g["launches"]},
         "breakdown_fp32_factor": breakdown,

@@ -97,6 +97,12 @@ __device__ __forceinline__ void record_status(int* status, int code, int front, 
 
 constexpr int LPAD = 16;
 
+// Three-multiplication (Gauss) complex products in the complex FP32 GEMM.
+#ifndef CLIQG_GEMM_3M
+#define CLIQG_GEMM_3M 1
+#endif
+constexpr bool GEMM_3M = CLIQG_GEMM_3M != 0;
+
 // number of column-major lower-trapezoid tiles before tile column tj
 __host__ __device__ __forceinline__ long long lower_prefix(long long tj, long long tilesM) {
     return tj * tilesM - tj * (tj - 1) / 2;
@@ -149,7 +155,7 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 }  // namespace
 
 int gemm_tile_m(int variant) { return variant == 0 ? 64 : 128; }
-int gemm_tile_n(int variant) { return (variant == 2 || variant == 3) ? 128 : 64; }
+int gemm_tile_n(int variant) { return (variant >= 2) ? 128 : 64; }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
@@ -172,7 +178,8 @@ int gemm_tiles(int M, int N, int flags, int variant) {
 // the MFMAs. Complex products use four real MFMAs on planar (re, im) tiles.
 // ---------------------------------------------------------------------------
 template <class T, int TM, int TN, int ROLE, int TK, int WM = 2, int WN = 2>
-__global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __restrict__ descs,
+__global__ __launch_bounds__(64 * WM * WN)
+__attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) void k_gemm(const GemmDesc<T>* __restrict__ descs,
                                                        const int* __restrict__ tile_start,
                                                        int ndesc) {
     using R = typename Scalar<T>::real;
@@ -208,13 +215,18 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
     const bool conjA = d.flags & G_CONJA, conjB = d.flags & G_CONJB;
     const W* scaleK = reinterpret_cast<const W*>(d.scaleK);
 
-    A4 accR[BM][BN], accI[BM][BN];
+    // Complex FP32 products use three real MFMAs per block (Gauss):
+    // P1 = sum aR bR, P2 = sum aI bI, P3 = sum (aR + aI)(bR + bI);
+    // Re = P1 - P2, Im = P3 - P1 - P2. Complex FP64 keeps four.
+    constexpr bool M3 = CPX && !F64 && GEMM_3M && BM * BN <= 4;   // registers: small wave tiles
+    A4 accR[BM][BN], accI[BM][BN], accS[M3 ? BM : 1][M3 ? BN : 1];
 #pragma unroll
     for (int a = 0; a < BM; ++a)
 #pragma unroll
         for (int b = 0; b < BN; ++b) {
             accR[a][b] = A4{0, 0, 0, 0};
             accI[a][b] = A4{0, 0, 0, 0};
+            if constexpr (M3) accS[a][b] = A4{0, 0, 0, 0};
         }
 
     // Fixed per-thread load geometry: element r of the A chunk is
@@ -351,17 +363,33 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
                 bR[q] = Bs[buf][0][kr][wj * WTN + q * 16 + (lane & 15)];
                 if constexpr (CPX) bI[q] = Bs[buf][NP - 1][kr][wj * WTN + q * 16 + (lane & 15)];
             }
+            if constexpr (M3) {
+                R sA[BM], sB[BN];
 #pragma unroll
-            for (int a = 0; a < BM; ++a)
+                for (int a = 0; a < BM; ++a) sA[a] = aR[a] + aI[a];
 #pragma unroll
-                for (int b = 0; b < BN; ++b) {
-                    accR[a][b] = mfma16(aR[a], bR[b], accR[a][b]);
-                    if constexpr (CPX) {
-                        accR[a][b] = mfma16(-aI[a], bI[b], accR[a][b]);
-                        accI[a][b] = mfma16(aR[a], bI[b], accI[a][b]);
-                        accI[a][b] = mfma16(aI[a], bR[b], accI[a][b]);
+                for (int b = 0; b < BN; ++b) sB[b] = bR[b] + bI[b];
+#pragma unroll
+                for (int a = 0; a < BM; ++a)
+#pragma unroll
+                    for (int b = 0; b < BN; ++b) {
+                        accR[a][b] = mfma16(aR[a], bR[b], accR[a][b]);   // P1
+                        accI[a][b] = mfma16(aI[a], bI[b], accI[a][b]);   // P2
+                        accS[a][b] = mfma16(sA[a], sB[b], accS[a][b]);   // P3
                     }
-                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < BM; ++a)
+#pragma unroll
+                    for (int b = 0; b < BN; ++b) {
+                        accR[a][b] = mfma16(aR[a], bR[b], accR[a][b]);
+                        if constexpr (CPX) {
+                            accR[a][b] = mfma16(-aI[a], bI[b], accR[a][b]);
+                            accI[a][b] = mfma16(aR[a], bI[b], accI[a][b]);
+                            accI[a][b] = mfma16(aI[a], bR[b], accI[a][b]);
+                        }
+                    }
+            }
         }
         if (c + 1 < nk) store(buf ^ 1);
         __syncthreads();
@@ -380,7 +408,12 @@ __global__ __launch_bounds__(64 * WM * WN) void k_gemm(const GemmDesc<T>* __rest
                 const int gi = row0 + wi * WTM + a * 16 + rin;
                 const int gj = col0 + wj * WTN + b * 16 + (lane & 15);
                 if (gi >= d.M || gj >= d.N || (lower && gi < gj)) continue;
-                T v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
+                T v;
+                if constexpr (M3)
+                    v = make_t<T>(accR[a][b][r] - accI[a][b][r],
+                                  accS[a][b][r] - accR[a][b][r] - accI[a][b][r]);
+                else
+                    v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
                 if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
                 if (neg) v = -v;
                 T* p = d.C + gi + (int64_t)gj * d.ldc;
@@ -420,6 +453,11 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
     } else if (variant == 3) {
         if constexpr (gemm_variant_allowed<T>(3))
             launch_gemm_shape<T, 128, 128, 16, 4, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+        else
+            throw Error(1, "128x128 GEMM tiles are not built for complex double");
+    } else if (variant == 4) {
+        if constexpr (gemm_variant_allowed<T>(4))
+            launch_gemm_shape<T, 128, 128, 16, 2, 2>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
             throw Error(1, "128x128 GEMM tiles are not built for complex double");
     } else {

@@ -11,8 +11,8 @@ namespace cliqg {
 // Width of a diagonal block (panel) of the partial LDL^T.
 constexpr int NB = 64;
 // Tile shapes of the grouped GEMM: variant 0 = 64x64 (4 waves), 1 = 128x64
-// (4 waves), 2 = 128x128 (8 waves), 3 = 128x128 (16 waves); 2 and 3 are not
-// built for complex double (LDS).
+// (4 waves), 2 = 128x128 (8 waves), 3 = 128x128 (16 waves), 4 = 128x128
+// (4 waves of 64x64); 2-4 are not built for complex double (LDS).
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
 template <class T>

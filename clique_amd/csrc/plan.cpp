@@ -14,6 +14,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -206,31 +207,32 @@ struct StepBuilder {
         steps.push_back(s);
     }
 
-    // Tile shape of a grouped launch: padded MFMA work divided by the share
-    // of the 256 CUs the launch can occupy, weighted by the measured relative
-    // cost per flop of each tile shape on gfx950 (the 128-row tiles run at one
-    // or two waves per SIMD and lose to 64x64 on every measured shape).
+    // Tile shape of a grouped launch: 64x64 tiles (four waves of 32x32) or
+    // 128x128 tiles (four waves of 64x64, two blocks per CU). Cost = rounds of
+    // 256 CUs the launch needs x MFMA work of one tile x the measured relative
+    // cost per flop of the shape on gfx950 (real FP64 0.83, real FP32 0.95 for
+    // the large tile). Complex FP32 always takes 64x64 tiles, whose
+    // three-multiplication products beat the large tile's four; complex FP64
+    // has no large tile (LDS).
     static int choose_variant(const std::vector<GemmDesc<T>>& in) {
-        static const double rel[3] = {1.0, 1.35, 1.6};
-        int best = 0;
-        double bestCost = 1e300;
-        bool lower = false;
-        for (const auto& d : in) lower |= (d.flags & G_LOWER) != 0;
-        for (int v = 0; v < 3; ++v) {
-            if (!gemm_variant_allowed<T>(v)) continue;
-            if (lower && gemm_tile_m(v) != gemm_tile_n(v)) continue;
-            double padded = 0.0, tiles = 0.0;
+        if (!gemm_variant_allowed<T>(4) || Scalar<T>::complex) return 0;
+        const double rel4 = sizeof(T) == 4 ? 0.95 : 0.83;
+        double cost[2] = {0.0, 0.0};
+        const int vs[2] = {0, 4};
+        for (int c = 0; c < 2; ++c) {
+            const int v = vs[c];
+            double tiles = 0.0, work = 0.0;
             for (const auto& d : in) {
                 const int t = gemm_tiles(d.M, d.N, d.flags, v);
                 tiles += t;
-                padded += (double)t * gemm_tile_m(v) * gemm_tile_n(v) * std::max(d.K, 1);
+                work += (double)t * std::max(d.K, 1);
             }
-            if (tiles == 0) continue;
-            const double occ = std::min(1.0, tiles / 256.0);
-            const double cost = rel[v] * padded / occ;
-            if (cost < bestCost) { bestCost = cost; best = v; }
+            if (tiles == 0) return 0;
+            // average K per tile times the tile area, times rounds over the CUs
+            const double rounds = std::ceil(tiles / 256.0);
+            cost[c] = (c ? rel4 : 1.0) * rounds * (work / tiles) * gemm_tile_m(v) * gemm_tile_n(v);
         }
-        return best;
+        return cost[1] < cost[0] ? 4 : 0;
     }
 };
 
@@ -789,7 +791,12 @@ struct Numeric : NumericBase {
                 hipEvent_t e1 = P.next_event();
                 CQ_HIP(hipEventRecord(e1, st));
                 const int kind = (s.kind == S_GEMM && S) ? 7 : s.kind;
-                P.pending.push_back(Plan::Pending{step_name(kind), s.flops, s.bytes, e0, e1});
+                // factor GEMMs by instantiation: pivot-scaled updates (Schur,
+                // panel and outer-panel updates) apart from the TRSM-like rest
+                const char* name = (s.kind == S_GEMM && !S)
+                                       ? (s.scaled ? "gemm_update" : "gemm_trsm")
+                                       : step_name(kind);
+                P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1});
             }
         }
         CQ_HIP(hipGetLastError());
