@@ -260,6 +260,10 @@ class DistSymmFrontTree:
         _lib.check(lib().cliqg_plan_memory(self._h, ctypes.byref(b)))
         return b.value
 
+    def set_graphs(self, enable=True):
+        """Replay the factor and solve launch sequences as HIP graphs (default off)."""
+        _lib.check(lib().cliqg_plan_set_graphs(self._h, int(enable)))
+
     def set_profiling(self, enable=True):
         _lib.check(lib().cliqg_plan_set_profiling(self._h, int(enable)))
 

@@ -49,6 +49,7 @@ EXPORTS = [
     "cliqg_spmv",
     "cliqg_plan_refine", "cliqg_plan_adjoint", "cliqg_plan_get_front",
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
+    "cliqg_plan_set_graphs",
 ]
 
 
@@ -96,6 +97,7 @@ def lib():
         "cliqg_plan_get_front": (c_int, [vp, c_int, vp, vp]),
         "cliqg_plan_last_factor_ms": (c_int, [vp, dp]),
         "cliqg_plan_set_profiling": (c_int, [vp, c_int]),
+        "cliqg_plan_set_graphs": (c_int, [vp, c_int]),
         "cliqg_plan_get_profile": (c_int, [vp, ctypes.POINTER(ProfileEntry), c_int, ip]),
     }
     for name, (res, args) in sig.items():

@@ -336,6 +336,10 @@ cliqg_status cliqg_plan_set_profiling(cliqg_plan plan, int enable) {
     });
 }
 
+cliqg_status cliqg_plan_set_graphs(cliqg_plan plan, int enable) {
+    return guarded([&] { P(plan).useGraphs = enable != 0; });
+}
+
 cliqg_status cliqg_plan_get_profile(cliqg_plan plan, cliqg_profile_entry* h_out, int capacity,
                                     int* count) {
     return guarded([&] {

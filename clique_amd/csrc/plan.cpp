@@ -177,12 +177,14 @@ struct StepBuilder {
     // triB: one operand is triangular (an inverted diagonal block, or a
     // triangular op(A) whose K range the kernel clamps), so only half of the
     // K products count as algorithmic work.
-    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) {
+    // variant >= 0 forces a tile shape (e.g. 128-row tiles for an in-place
+    // product whose rows must all sit in one tile).
+    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false, int forced = -1) {
         std::vector<GemmDesc<T>> ds;
         std::vector<int> start;
         int total = 0;
         double flops = 0.0, bytes = 0.0;
-        const int variant = choose_variant(in);
+        const int variant = forced >= 0 ? forced : choose_variant(in);
         for (const auto& d : in) {
             const int t = gemm_tiles(d.M, d.N, d.flags, variant);
             if (t == 0) continue;
@@ -243,6 +245,8 @@ struct StepBuilder {
 // ---------------------------------------------------------------------------
 // Outer panel width of the front factorization (a multiple of NB).
 constexpr int OB = 256;
+// 128 x 64 tiles: in-place solve products over at most 128 rows.
+constexpr int INPLACE_VARIANT = 1;
 
 template <class T>
 struct Numeric : NumericBase {
@@ -256,11 +260,23 @@ struct Numeric : NumericBase {
     // A21, the unit-lower inverses of L11 and the n x 64 product panels
     DevBuf bsave, bsinv, btmp;
 
+    // A captured launch sequence (HIP graph), replayed instead of the
+    // individual launches.
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        bool tried = false;
+        ~Graph() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+        }
+    };
+
     struct SolveSched {
         Arena arena;
         std::vector<Step> fwd, bwd;
         DevBuf wpool, xn, tmp;
+        Graph graph;   // forward, diagonal and backward sweeps
     };
+    Graph fgraph[2];   // LDL and block-LDL factorizations
     std::map<int, std::unique_ptr<SolveSched>> solves;
     DevBuf rbuf, identity;
     int rbufCols = 0;
@@ -567,8 +583,7 @@ struct Numeric : NumericBase {
         SolveSched& S = *sp;
         S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
         S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
-        // per-front k-column workspace: n rows for block fronts, one wide
-        // panel (at most SB rows) for LDL fronts
+        // per-front n x k workspace of the block solves
         std::vector<int64_t> tmpOff(P.numFronts, 0);
         {
             int64_t mx = 0;
@@ -576,7 +591,7 @@ struct Numeric : NumericBase {
                 int64_t a = 0;
                 for (int s : P.levels[h]) {
                     tmpOff[s] = a;
-                    a += (int64_t)(blk ? P.fronts[s].n : std::min(P.fronts[s].n, SB)) * k;
+                    a += (int64_t)(blk ? P.fronts[s].n : 0) * k;
                 }
                 mx = std::max(mx, a);
             }
@@ -645,28 +660,26 @@ struct Numeric : NumericBase {
                         sb.copy(cs);
                         sb.gemm(g);
                     }
-                    // SB-wide panels: Y := inv(L_pp) X_p (wide inverse, out of
-                    // place), X(below) -= L(below, p) Y, X_p := Y
+                    // SB-wide panels: X_p := inv(L_pp) X_p, X(below) -= L(below, p) X_p.
+                    // The first product is in place: with 128-row tiles all of a
+                    // panel's rows sit in one tile, which reads its whole K range
+                    // before it writes.
                     for (int w0 = 0; w0 < (blk ? 0 : maxn); w0 += SB) {
                         std::vector<GemmDesc<T>> tr, up;
-                        std::vector<CopyDesc<T>> cp;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
                             if (f.n <= w0) continue;
                             const int sbw = std::min(SB, f.n - w0);
                             T* x = Xs(s) + w0;
-                            T* y = Tm(s);
-                            tr.push_back(GemmDesc<T>{LW(s, w0), x, y, nullptr, nullptr, sbw, N, sbw,
+                            tr.push_back(GemmDesc<T>{LW(s, w0), x, x, nullptr, nullptr, sbw, N, N,
                                                      sbw, k, sbw, G_KHI});
                             if (f.n - w0 - sbw > 0)
-                                up.push_back(GemmDesc<T>{L(s) + (w0 + sbw) + (int64_t)w0 * f.m, y,
-                                                         x + sbw, nullptr, nullptr, f.m, sbw, N,
+                                up.push_back(GemmDesc<T>{L(s) + (w0 + sbw) + (int64_t)w0 * f.m, x,
+                                                         x + sbw, nullptr, nullptr, f.m, N, N,
                                                          f.n - w0 - sbw, k, sbw, G_ACCUM | G_NEG});
-                            cp.push_back(CopyDesc<T>{y, x, sbw, N, sbw, k});
                         }
-                        sb.gemm(tr, true);
+                        sb.gemm(tr, true, INPLACE_VARIANT);
                         sb.gemm(up);
-                        sb.copy(cp);
                     }
                     // the front's update rows in one product with K = n
                     std::vector<GemmDesc<T>> g;
@@ -713,30 +726,26 @@ struct Numeric : NumericBase {
                         continue;
                     }
                     sb.gemm(g);
-                    // SB-wide panels from the last: Y := inv(L_pp)^T X_p (^H if
-                    // Hermitian), X(0:w0) -= L(p, 0:w0)^T Y, X_p := Y
+                    // SB-wide panels from the last: X_p := inv(L_pp)^T X_p (^H if
+                    // Hermitian, in place as above), X(0:w0) -= L(p, 0:w0)^T X_p
                     const int maxw = (maxn + SB - 1) / SB;
                     for (int pw = maxw - 1; pw >= 0; --pw) {
                         const int w0 = pw * SB;
                         std::vector<GemmDesc<T>> tr, up;
-                        std::vector<CopyDesc<T>> cp;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
                             if (f.n <= w0) continue;
                             const int sbw = std::min(SB, f.n - w0);
                             T* x = Xs(s) + w0;
-                            T* y = Tm(s);
-                            tr.push_back(GemmDesc<T>{LW(s, w0), x, y, nullptr, nullptr, sbw, N, sbw,
+                            tr.push_back(GemmDesc<T>{LW(s, w0), x, x, nullptr, nullptr, sbw, N, N,
                                                      sbw, k, sbw, G_TRANSA | G_KLO | cA});
                             if (w0 > 0)
-                                up.push_back(GemmDesc<T>{L(s) + w0, y, Xs(s), nullptr, nullptr, f.m,
-                                                         sbw, N, w0, k, sbw,
+                                up.push_back(GemmDesc<T>{L(s) + w0, x, Xs(s), nullptr, nullptr, f.m,
+                                                         N, N, w0, k, sbw,
                                                          G_TRANSA | G_ACCUM | G_NEG | cA});
-                            cp.push_back(CopyDesc<T>{y, x, sbw, N, sbw, k});
                         }
-                        sb.gemm(tr, true);
+                        sb.gemm(tr, true, INPLACE_VARIANT);
                         sb.gemm(up);
-                        sb.copy(cp);
                     }
                 }
             }
@@ -802,6 +811,45 @@ struct Numeric : NumericBase {
         CQ_HIP(hipGetLastError());
     }
 
+    // Record body's launches on the plan's stream into g (once). Capture
+    // needs a stream other than the legacy default one; where it fails the
+    // launches run directly. No memset is recorded (the graphs hold kernels
+    // only).
+    template <class F>
+    bool replay(Graph& g, F&& body) {
+        if (!P.useGraphs || P.profiling) return false;
+        if (!g.tried) {
+            g.tried = true;
+            hipStream_t st = P.stream;
+            if (hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+                (void)hipGetLastError();
+                return false;
+            }
+            hipGraph_t graph = nullptr;
+            try {
+                body();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            if (hipStreamEndCapture(st, &graph) != hipSuccess || !graph) {
+                (void)hipGetLastError();
+                return false;
+            }
+            const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                g.exec = nullptr;
+                return false;
+            }
+        }
+        if (!g.exec) return false;
+        CQ_HIP(hipGraphLaunch(g.exec, P.stream));
+        return true;
+    }
+
     void factor(const V* vals) {
         hipStream_t st = P.stream;
         if (block() && !blockBuilt) {
@@ -809,11 +857,15 @@ struct Numeric : NumericBase {
             blockBuilt = true;
         }
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
-        launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate, st);
-        if (block())
-            run(bsteps, barena, nullptr, 0);
-        else
-            run(fsteps, farena, nullptr, 0);
+        auto body = [&] {
+            launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate,
+                                  st);
+            if (block())
+                run(bsteps, barena, nullptr, 0);
+            else
+                run(fsteps, farena, nullptr, 0);
+        };
+        if (!replay(fgraph[block() ? 1 : 0], body)) body();
     }
 
     // x: V, N x k (ld ldx); nodal or original ordering; solved in place
@@ -830,13 +882,16 @@ struct Numeric : NumericBase {
             map = identity.as<int>();
         }
         launch_pull<T, V>(x, ldx, map, S.xn.template as<T>(), P.N, k, st);
-        run(S.fwd, S.arena, &S, k);
-        // block fronts hold inv(ATL), so there is no separate diagonal solve
-        // (solve.hpp:48-53)
-        if (!block())
-            launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k,
-                                    P.dStatus.as<int>(), st);
-        run(S.bwd, S.arena, &S, k);
+        auto body = [&] {
+            run(S.fwd, S.arena, &S, k);
+            // block fronts hold inv(ATL), so there is no separate diagonal
+            // solve (solve.hpp:48-53)
+            if (!block())
+                launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k,
+                                        P.dStatus.as<int>(), st);
+            run(S.bwd, S.arena, &S, k);
+        };
+        if (!replay(S.graph, body)) body();
         launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
         CQ_HIP(hipGetLastError());
     }
@@ -876,7 +931,8 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     N = S.N;
     if (const char* e = std::getenv("CLIQG_SOLVE_BLOCK")) {
         const int v = std::atoi(e);
-        if (v < NB || v % NB) throw Error(1, "CLIQG_SOLVE_BLOCK must be a positive multiple of 64");
+        if (v < NB || v % NB || v > 128)
+            throw Error(1, "CLIQG_SOLVE_BLOCK must be 64 or 128");
         sb = v;
     }
     numFronts = (int)S.nodes.size();
@@ -985,6 +1041,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     }
     CQ_HIP(hipEventCreate(&ev0_));
     CQ_HIP(hipEventCreate(&ev1_));
+    CQ_HIP(hipEventCreateWithFlags(&evCaller_, hipEventDisableTiming));
     upload(dRowOff, hostRowOff);
     std::vector<int> colv(colp, colp + nnz);
     upload(dCols, colv);
@@ -1002,7 +1059,14 @@ Plan::~Plan() {
     for (hipEvent_t e : evPool_) (void)hipEventDestroy(e);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (evCaller_) (void)hipEventDestroy(evCaller_);
     if (ownStream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Plan::after_caller() {
+    if (!ownStream) return;   // the caller's stream orders the work itself
+    CQ_HIP(hipEventRecord(evCaller_, nullptr));
+    CQ_HIP(hipStreamWaitEvent(stream, evCaller_, 0));
 }
 
 size_t Plan::device_bytes() const {
@@ -1056,6 +1120,7 @@ static void dispatch(bool complex, int precision, F&& f) {
 }
 
 void Plan::factor(const void* d_vals, int prec, int ftype) {
+    after_caller();
     if (prec != 0 && prec != 1) throw Error(1, "unknown precision");
     if (ftype != 0 && ftype != 1) throw Error(1, "unknown front type");
     if (!num || prec != precision) {
@@ -1098,6 +1163,7 @@ void Plan::factor(const void* d_vals, int prec, int ftype) {
 }
 
 void Plan::solve(void* d_x, int64_t ldx, int k, bool nodal) {
+    after_caller();
     if (!factored) throw Error(6, "the plan has not been factored");
     if (k <= 0) return;
     if (ldx < N) throw Error(1, "leading dimension smaller than N");
@@ -1111,6 +1177,7 @@ void Plan::solve(void* d_x, int64_t ldx, int k, bool nodal) {
 
 void Plan::multiply(double alpha, const void* d_x, int64_t ldx, double beta, void* d_y,
                     int64_t ldy, int k) {
+    after_caller();
     if (!factored) throw Error(6, "the plan holds no matrix values yet");
     if (complex)
         launch_spmv<cdouble>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<cdouble>(), alpha,
@@ -1119,10 +1186,16 @@ void Plan::multiply(double alpha, const void* d_x, int64_t ldx, double beta, voi
         launch_spmv<double>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<double>(), alpha,
                             (const double*)d_x, ldx, beta, (double*)d_y, ldy, k, stream);
     CQ_HIP(hipGetLastError());
+    if (ownStream) {
+        // the caller's later work on the default stream reads y
+        CQ_HIP(hipEventRecord(evCaller_, stream));
+        CQ_HIP(hipStreamWaitEvent(nullptr, evCaller_, 0));
+    }
 }
 
 void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, double tol,
                   int maxIters, int* itersOut, int* convOut, double* etaOut) {
+    after_caller();
     if (!factored) throw Error(6, "the plan has not been factored");
     if (k <= 0) return;
     if (k > 1000) throw Error(1, "at most 1000 right-hand sides per refinement call");
@@ -1178,6 +1251,7 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
 
 void Plan::adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, int k,
                    void* d_lambda, int64_t ldl, void* d_dA) {
+    after_caller();
     if (!factored) throw Error(6, "the plan has not been factored");
     if (k <= 0) return;
     if (precision == 1) {

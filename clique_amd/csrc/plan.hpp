@@ -103,6 +103,7 @@ public:
         double ms = 0.0, flops = 0.0, bytes = 0.0;
     };
     bool profiling = false;
+    bool useGraphs = false;  // replay the factor and solve launch sequences as HIP graphs (opt-in)
     std::vector<Pending> pending;
     std::map<std::string, ProfileEntry> profile;
     hipEvent_t next_event();
@@ -110,7 +111,10 @@ public:
 
 private:
     double lastFactorMs_ = 0.0;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, evCaller_ = nullptr;
+    // Order the plan's own stream after the caller's prior work on the
+    // default stream (the plan creates a non-blocking stream when given none).
+    void after_caller();
     std::vector<hipEvent_t> evPool_;
     size_t evNext_ = 0;
 };

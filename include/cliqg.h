@@ -197,6 +197,14 @@ cliqg_status cliqg_plan_set_profiling(cliqg_plan plan, int enable);
 cliqg_status cliqg_plan_get_profile(cliqg_plan plan, cliqg_profile_entry* h_out, int capacity,
                                     int* count);
 
+/* Launch replay (no reference counterpart; Clique has no GPU path): with
+ * enable = 1 the factorization's and each solve width's launch sequences are
+ * recorded once as HIP graphs on the plan's stream and replayed; the default
+ * (0) launches every kernel individually. Results are identical. Capture is
+ * skipped while profiling and on the legacy default stream. On MI355X the
+ * replay measured no faster than direct launches for the 120^3 workload. */
+cliqg_status cliqg_plan_set_graphs(cliqg_plan plan, int enable);
+
 #ifdef __cplusplus
 }
 #endif

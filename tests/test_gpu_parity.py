@@ -306,3 +306,32 @@ def test_refactor_switches_between_ldl_and_block(device):
         xs.append(x.cpu().numpy())
     assert rel_diff(xs[1], xs[0]) <= 1e-12
     assert np.array_equal(xs[2], xs[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_graph_replays_match_direct_launches(device, precision):
+    """The factor and solve launch sequences are recorded once as HIP graphs
+    and replayed; every replay (with host work and synchronizations between
+    them, and a refactorization with new values) must equal, bit for bit,
+    the same plan type launching each kernel directly."""
+    ip, ind, v, dims = problem("pml", 14)
+    N = ip.size - 1
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=32)
+    g = cq.DistSymmFrontTree(sym, ip, ind, is_complex=True)
+    d = cq.DistSymmFrontTree(sym, ip, ind, is_complex=True)
+    g.set_graphs(True)
+    d.set_graphs(False)
+    rng = np.random.default_rng(21)
+    for scale in (1.0, 1.5):
+        vals = torch.tensor(v * scale, device="cuda:0")
+        g.factor(vals, precision=precision)
+        d.factor(vals, precision=precision)
+        for s in (0, sym.num_nodes // 2, sym.num_nodes - 1):
+            assert np.array_equal(g.front(s)[0], d.front(s)[0])
+        for rep in range(3):
+            b = rng.standard_normal((N, 5)) + 1j * rng.standard_normal((N, 5))
+            xg, xd = to_dev(b), to_dev(b)
+            g.solve(xg)
+            d.solve(xd)
+            assert np.array_equal(xg.cpu().numpy(), xd.cpu().numpy()), (scale, rep)
