@@ -333,6 +333,9 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
     for (int c = 0; c < nk; ++c) {
         const int buf = c & 1;
         if (c + 1 < nk) load(kbeg + (c + 1) * TK);
+        // raised wave priority while feeding the MFMAs (measured: +3 % for
+        // complex FP64, neutral elsewhere)
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
@@ -391,6 +394,7 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
                     }
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         if (c + 1 < nk) store(buf ^ 1);
         __syncthreads();
     }
@@ -578,11 +582,11 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 //     a(i,c)  -= a(i,j) op(a(c,j)) / d_j          (lower, c > j)
 //     li(q,i) -= (a(q,j) / d_j) li(j,i)           (upper, i <= j < q; li(j,j) = 1)
 // ---------------------------------------------------------------------------
+// (Two waves per SIMD are plenty for this latency-bound kernel; the complex
+// instantiations need the registers.)
 template <class T>
-// (two waves per SIMD are plenty for a latency-bound kernel; the complex
-// instantiations need the registers: at the default budget they spill)
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_diag_ldl(const DiagDesc<T>* __restrict__ descs,
-                                                  bool herm, int* status) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_diag_ldl(
+    const DiagDesc<T>* __restrict__ descs, bool herm, int* status) {
     using W = typename Scalar<T>::wide;
     __shared__ W col[2][NB];
     __shared__ W dinv[NB];

@@ -245,8 +245,6 @@ struct StepBuilder {
 // ---------------------------------------------------------------------------
 // Outer panel width of the front factorization (a multiple of NB).
 constexpr int OB = 256;
-// 128 x 64 tiles: in-place solve products over at most 128 rows.
-constexpr int INPLACE_VARIANT = 1;
 
 template <class T>
 struct Numeric : NumericBase {
@@ -273,7 +271,7 @@ struct Numeric : NumericBase {
     struct SolveSched {
         Arena arena;
         std::vector<Step> fwd, bwd;
-        DevBuf wpool, xn, tmp;
+        DevBuf wpool, xn, xn2, tmp;
         Graph graph;   // forward, diagonal and backward sweeps
     };
     Graph fgraph[2];   // LDL and block-LDL factorizations
@@ -583,6 +581,9 @@ struct Numeric : NumericBase {
         SolveSched& S = *sp;
         S.wpool.alloc(std::max<int64_t>(P.wRows * k, 1) * sizeof(T));
         S.xn.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
+        // LDL sweeps ping-pong between xn and xn2: the forward sweep takes its
+        // input in xn and leaves X in xn2, the backward sweep the reverse
+        if (!blk) S.xn2.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
         // per-front n x k workspace of the block solves
         std::vector<int64_t> tmpOff(P.numFronts, 0);
         {
@@ -604,6 +605,8 @@ struct Numeric : NumericBase {
         const int64_t N = P.N;
         auto Wp = [&](int s) { return wp + P.fronts[s].W * k; };
         auto Xs = [&](int s) { return xn + P.fronts[s].off; };
+        T* xn2 = blk ? xn : S.xn2.template as<T>();
+        auto X2 = [&](int s) { return xn2 + P.fronts[s].off; };
         const int cA = P.conjugate ? G_CONJA : 0;
         for (int pass = 0; pass < 2; ++pass) {
             const bool bwd = pass == 1;
@@ -660,10 +663,8 @@ struct Numeric : NumericBase {
                         sb.copy(cs);
                         sb.gemm(g);
                     }
-                    // SB-wide panels: X_p := inv(L_pp) X_p, X(below) -= L(below, p) X_p.
-                    // The first product is in place: with 128-row tiles all of a
-                    // panel's rows sit in one tile, which reads its whole K range
-                    // before it writes.
+                    // SB-wide panels: X2_p := inv(L_pp) X_p (into the second
+                    // buffer), X(below) -= L(below, p) X2_p
                     for (int w0 = 0; w0 < (blk ? 0 : maxn); w0 += SB) {
                         std::vector<GemmDesc<T>> tr, up;
                         for (int s : lv) {
@@ -671,14 +672,15 @@ struct Numeric : NumericBase {
                             if (f.n <= w0) continue;
                             const int sbw = std::min(SB, f.n - w0);
                             T* x = Xs(s) + w0;
-                            tr.push_back(GemmDesc<T>{LW(s, w0), x, x, nullptr, nullptr, sbw, N, N,
+                            T* y = X2(s) + w0;
+                            tr.push_back(GemmDesc<T>{LW(s, w0), x, y, nullptr, nullptr, sbw, N, N,
                                                      sbw, k, sbw, G_KHI});
                             if (f.n - w0 - sbw > 0)
-                                up.push_back(GemmDesc<T>{L(s) + (w0 + sbw) + (int64_t)w0 * f.m, x,
+                                up.push_back(GemmDesc<T>{L(s) + (w0 + sbw) + (int64_t)w0 * f.m, y,
                                                          x + sbw, nullptr, nullptr, f.m, N, N,
                                                          f.n - w0 - sbw, k, sbw, G_ACCUM | G_NEG});
                         }
-                        sb.gemm(tr, true, INPLACE_VARIANT);
+                        sb.gemm(tr, true);
                         sb.gemm(up);
                     }
                     // the front's update rows in one product with K = n
@@ -686,7 +688,7 @@ struct Numeric : NumericBase {
                     for (int s : lv) {
                         const FrontLayout& f = P.fronts[s];
                         if (f.u == 0 || f.n == 0) continue;
-                        g.push_back(GemmDesc<T>{L(s) + f.n, Xs(s), Wp(s), nullptr, nullptr, f.m, N,
+                        g.push_back(GemmDesc<T>{L(s) + f.n, X2(s), Wp(s), nullptr, nullptr, f.m, N,
                                                 f.u, f.u, k, f.n, G_ACCUM | G_NEG});
                     }
                     sb.gemm(g);
@@ -702,7 +704,7 @@ struct Numeric : NumericBase {
                         rs.push_back(rows);
                         rows += f.u;
                         if (f.n > 0)
-                            g.push_back(GemmDesc<T>{L(s) + f.n, Wp(s), Xs(s), nullptr, nullptr, f.m,
+                            g.push_back(GemmDesc<T>{L(s) + f.n, Wp(s), X2(s), nullptr, nullptr, f.m,
                                                     f.u, N, f.n, k, f.u,
                                                     G_TRANSA | G_ACCUM | G_NEG | cA});
                     }
@@ -726,8 +728,8 @@ struct Numeric : NumericBase {
                         continue;
                     }
                     sb.gemm(g);
-                    // SB-wide panels from the last: X_p := inv(L_pp)^T X_p (^H if
-                    // Hermitian, in place as above), X(0:w0) -= L(p, 0:w0)^T X_p
+                    // SB-wide panels from the last: X_p := inv(L_pp)^T X2_p (^H if
+                    // Hermitian; back into xn), X2(0:w0) -= L(p, 0:w0)^T X_p
                     const int maxw = (maxn + SB - 1) / SB;
                     for (int pw = maxw - 1; pw >= 0; --pw) {
                         const int w0 = pw * SB;
@@ -737,14 +739,15 @@ struct Numeric : NumericBase {
                             if (f.n <= w0) continue;
                             const int sbw = std::min(SB, f.n - w0);
                             T* x = Xs(s) + w0;
-                            tr.push_back(GemmDesc<T>{LW(s, w0), x, x, nullptr, nullptr, sbw, N, N,
+                            T* y = X2(s) + w0;
+                            tr.push_back(GemmDesc<T>{LW(s, w0), y, x, nullptr, nullptr, sbw, N, N,
                                                      sbw, k, sbw, G_TRANSA | G_KLO | cA});
                             if (w0 > 0)
-                                up.push_back(GemmDesc<T>{L(s) + w0, x, Xs(s), nullptr, nullptr, f.m,
+                                up.push_back(GemmDesc<T>{L(s) + w0, x, X2(s), nullptr, nullptr, f.m,
                                                          N, N, w0, k, sbw,
                                                          G_TRANSA | G_ACCUM | G_NEG | cA});
                         }
-                        sb.gemm(tr, true, INPLACE_VARIANT);
+                        sb.gemm(tr, true);
                         sb.gemm(up);
                     }
                 }
@@ -887,7 +890,7 @@ struct Numeric : NumericBase {
             // block fronts hold inv(ATL), so there is no separate diagonal
             // solve (solve.hpp:48-53)
             if (!block())
-                launch_diag_solve<T, V>(S.xn.template as<T>(), dpool.as<V>(), P.N, k,
+                launch_diag_solve<T, V>(S.xn2.template as<T>(), dpool.as<V>(), P.N, k,
                                         P.dStatus.as<int>(), st);
             run(S.bwd, S.arena, &S, k);
         };
@@ -931,8 +934,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     N = S.N;
     if (const char* e = std::getenv("CLIQG_SOLVE_BLOCK")) {
         const int v = std::atoi(e);
-        if (v < NB || v % NB || v > 128)
-            throw Error(1, "CLIQG_SOLVE_BLOCK must be 64 or 128");
+        if (v < NB || v % NB) throw Error(1, "CLIQG_SOLVE_BLOCK must be a positive multiple of 64");
         sb = v;
     }
     numFronts = (int)S.nodes.size();
