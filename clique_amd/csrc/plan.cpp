@@ -109,7 +109,8 @@ struct IntervalAlloc {
     }
 };
 
-enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY };
+enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
+                S_RECORD, S_WAIT };
 
 struct Step {
     int kind;
@@ -120,6 +121,8 @@ struct Step {
     double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
     int variant = 0;      // GEMM tile shape
     bool scaled = false;  // GEMM descriptors carry pivots (scaleK)
+    int stream = 0;       // 0: the plan's stream, 1: its side stream
+    int event = -1;       // S_RECORD / S_WAIT: index into the plan's dependency events
 };
 
 const char* step_name(int kind) {
@@ -140,6 +143,21 @@ template <class T>
 struct StepBuilder {
     Arena& arena;
     std::vector<Step>& steps;
+    int cur = 0;   // stream of the steps added next
+
+    void on(int stream) { cur = stream; }
+    void record(int ev) {
+        Step x{S_RECORD, 0, 0, 0, 0, false};
+        x.stream = cur;
+        x.event = ev;
+        steps.push_back(x);
+    }
+    void wait(int ev) {
+        Step x{S_WAIT, 0, 0, 0, 0, false};
+        x.stream = cur;
+        x.event = ev;
+        steps.push_back(x);
+    }
 
     void zero(const std::vector<ZeroSeg<T>>& zs) {
         std::vector<int> start;
@@ -154,6 +172,7 @@ struct StepBuilder {
         if (keep.empty()) return;
         steps.push_back(Step{S_ZERO, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
                              false});
+        steps.back().stream = cur;
     }
 
     void copy(const std::vector<CopyDesc<T>>& cs) {
@@ -172,6 +191,7 @@ struct StepBuilder {
         if (keep.empty()) return;
         steps.push_back(Step{S_COPY, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
                              false, 0.0, by});
+        steps.back().stream = cur;
     }
 
     // triB: one operand is triangular (an inverted diagonal block, or a
@@ -206,6 +226,7 @@ struct StepBuilder {
         Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false, flops, bytes};
         s.variant = variant;
         s.scaled = scaled;
+        s.stream = cur;
         steps.push_back(s);
     }
 
@@ -253,6 +274,7 @@ struct Numeric : NumericBase {
     DevBuf fpool, upool, dpool, linv, linvw, wtmp;
     Arena farena, barena;
     std::vector<Step> fsteps, bsteps;   // LDL and block-LDL factor schedules
+    int nextEvent = 0;                  // dependency events used by the schedules
     bool blockBuilt = false;
     // block LDL work buffers, reused level by level: the fronts' original
     // A21, the unit-lower inverses of L11 and the n x 64 product panels
@@ -296,6 +318,7 @@ struct Numeric : NumericBase {
             wtmp.alloc(std::max<int64_t>(mx, 1) * sizeof(T));
         }
         build_factor_schedule(farena, fsteps, false);
+        P.ensure_dep_events(nextEvent);
     }
 
     bool block() const { return P.frontType == 1; }
@@ -402,6 +425,7 @@ struct Numeric : NumericBase {
             }
             int maxn = 0;
             for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
+            int pendingB2 = -1;   // side-stream update the next panel's B1 must wait for
             // Two-level panels: 64-wide diagonal blocks whose updates stay
             // inside an OB-wide outer panel, then one rank-OB update of the
             // front's trailing columns (K = OB instead of K = 64 passes).
@@ -437,18 +461,48 @@ struct Numeric : NumericBase {
                     sb.gemm(trsm, true);
                     sb.gemm(upd);
                 }
-                std::vector<GemmDesc<T>> outer;
+                // Outer update with a lookahead of one panel: the next outer
+                // panel's columns (B1) on the plan's stream, the columns after
+                // it (B2) on the side stream, overlapping the next panel's
+                // diagonal blocks and TRSMs. B2 of this panel must finish
+                // before B1 of the next one writes the same columns.
+                std::vector<GemmDesc<T>> b1, b2;
                 for (int s : lv) {
                     const FrontLayout& f = P.fronts[s];
                     const int c1 = std::min(f.n, k0 + OB);
                     if (f.n <= k0 || f.n - c1 <= 0) continue;
-                    T* l = L(s) + c1 + (int64_t)k0 * f.m;
-                    T* a22 = L(s) + c1 + (int64_t)c1 * f.m;
-                    outer.push_back(GemmDesc<T>{l, l, a22, D(s) + k0, nullptr, f.m, f.m, f.m, f.m - c1,
-                                                f.n - c1, c1 - k0,
-                                                G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                    const int c2 = std::min(f.n, c1 + OB);
+                    T* l1 = L(s) + c1 + (int64_t)k0 * f.m;
+                    b1.push_back(GemmDesc<T>{l1, l1, L(s) + c1 + (int64_t)c1 * f.m, D(s) + k0, nullptr,
+                                             f.m, f.m, f.m, f.m - c1, c2 - c1, c1 - k0,
+                                             G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                    if (f.n - c2 > 0) {
+                        T* l2 = L(s) + c2 + (int64_t)k0 * f.m;
+                        b2.push_back(GemmDesc<T>{l2, l2, L(s) + c2 + (int64_t)c2 * f.m, D(s) + k0,
+                                                 nullptr, f.m, f.m, f.m, f.m - c2, f.n - c2, c1 - k0,
+                                                 G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                    }
                 }
-                sb.gemm(outer);
+                int evB2 = -1;
+                if (!b2.empty()) {
+                    // B2 needs this panel's final L columns, and runs after
+                    // the previous B2 on the side stream
+                    const int evL = nextEvent++;
+                    evB2 = nextEvent++;
+                    sb.record(evL);
+                    sb.on(1);
+                    sb.wait(evL);
+                    sb.gemm(b2);
+                    sb.record(evB2);
+                    sb.on(0);
+                }
+                if (pendingB2 >= 0) sb.wait(pendingB2);
+                sb.gemm(b1);
+                pendingB2 = evB2;
+            }
+            if (pendingB2 >= 0) {
+                sb.wait(pendingB2);
+                pendingB2 = -1;
             }
             std::vector<GemmDesc<T>> syrk;
             for (int s : lv) {
@@ -760,8 +814,16 @@ struct Numeric : NumericBase {
     }
 
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
-        hipStream_t st = P.stream;
         for (const Step& s : steps) {
+            hipStream_t st = s.stream ? P.stream2 : P.stream;
+            if (s.kind == S_RECORD) {
+                CQ_HIP(hipEventRecord(P.depEvents[s.event], st));
+                continue;
+            }
+            if (s.kind == S_WAIT) {
+                CQ_HIP(hipStreamWaitEvent(st, P.depEvents[s.event], 0));
+                continue;
+            }
             hipEvent_t e0 = nullptr;
             if (P.profiling) {
                 e0 = P.next_event();
@@ -857,6 +919,7 @@ struct Numeric : NumericBase {
         hipStream_t st = P.stream;
         if (block() && !blockBuilt) {
             build_factor_schedule(barena, bsteps, true);
+            P.ensure_dep_events(nextEvent);
             blockBuilt = true;
         }
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
@@ -1044,6 +1107,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     CQ_HIP(hipEventCreate(&ev0_));
     CQ_HIP(hipEventCreate(&ev1_));
     CQ_HIP(hipEventCreateWithFlags(&evCaller_, hipEventDisableTiming));
+    CQ_HIP(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
     upload(dRowOff, hostRowOff);
     std::vector<int> colv(colp, colp + nnz);
     upload(dCols, colv);
@@ -1062,7 +1126,17 @@ Plan::~Plan() {
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     if (evCaller_) (void)hipEventDestroy(evCaller_);
+    for (hipEvent_t e : depEvents) (void)hipEventDestroy(e);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (ownStream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Plan::ensure_dep_events(int n) {
+    while ((int)depEvents.size() < n) {
+        hipEvent_t e;
+        CQ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        depEvents.push_back(e);
+    }
 }
 
 void Plan::after_caller() {

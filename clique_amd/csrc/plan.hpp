@@ -83,6 +83,11 @@ public:
     std::vector<int64_t> hostRowOff;
     hipStream_t stream = nullptr;
     bool ownStream = false;
+    // side stream (default priority) for work that overlaps the main
+    // sequence, and the events that order the two
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> depEvents;
+    void ensure_dep_events(int n);
 
     DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dInvMap, dVals, dStatus, dNorms;
     int precision = -1, frontType = 0;
