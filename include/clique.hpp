@@ -781,21 +781,76 @@ public:
     int Height() const { return h_; }
     int Width() const { return w_; }
     void* DeviceBuffer() const { return buf_.get(); }
+    // copy of another nodal multi-vector (device to device)
+    void Assign(const DistNodalMultiVec<F>& X) {
+        h_ = X.h_;
+        w_ = X.w_;
+        const size_t bytes = (size_t)h_ * w_ * sizeof(F);
+        buf_.resize(bytes);
+        if (bytes)
+            detail::check(cliqg_memcpy(buf_.get(), X.buf_.get(), bytes, 2), "DistNodalMultiVec copy");
+    }
 
 private:
     int h_ = 0, w_ = 0;
     detail::DeviceBuffer buf_;
 };
 
-// (numeric/solve.hpp:40-92)
+// (numeric/dist_nodal_matrix/decl.hpp:19-48) The right-hand sides for 2D
+// front types. At world size 1 the [MC,MR] distribution of each node's block
+// is the whole block, so the storage is the nodal multi-vector's: one
+// device-resident N x k block in nodal order.
+template <typename F>
+class DistNodalMatrix {
+public:
+    DistNodalMatrix() = default;
+    DistNodalMatrix(const DistMap& inverseMap, const DistSymmInfo& info,
+                    const DistMultiVec<F>& X) {
+        Pull(inverseMap, info, X);
+    }
+    explicit DistNodalMatrix(const DistNodalMultiVec<F>& X) { *this = X; }
+    const DistNodalMatrix<F>& operator=(const DistNodalMultiVec<F>& X) {
+        v_.Assign(X);
+        return *this;
+    }
+    void Pull(const DistMap& inverseMap, const DistSymmInfo& info, const DistMultiVec<F>& X) {
+        v_.Pull(inverseMap, info, X);
+    }
+    void Push(const DistMap& inverseMap, const DistSymmInfo& info, DistMultiVec<F>& X) const {
+        v_.Push(inverseMap, info, X);
+    }
+    int Height() const { return v_.Height(); }
+    int Width() const { return v_.Width(); }
+    void* DeviceBuffer() const { return v_.DeviceBuffer(); }
+
+private:
+    DistNodalMultiVec<F> v_;
+};
+
+namespace detail {
+template <typename F>
+inline void solve_nodal(const DistSymmFrontTree<F>& L, void* buf, int h, int w) {
+    if (Unfactored(L.frontType)) throw LogicError("Nonsensical front type for solve");
+    detail::check(cliqg_plan_solve_nodal(L.Plan(), buf, std::max(h, 1), w), "Solve");
+}
+}  // namespace detail
+
+// (numeric/solve.hpp:40-92) 1D front types solve against a
+// DistNodalMultiVec, 2D types against a DistNodalMatrix; block front types
+// skip the diagonal solve inside the engine.
 template <typename F>
 inline void Solve(const DistSymmInfo& info, const DistSymmFrontTree<F>& L,
                   DistNodalMultiVec<F>& X) {
     (void)info;
-    if (Unfactored(L.frontType)) throw LogicError("Nonsensical front type for solve");
-    detail::check(cliqg_plan_solve_nodal(L.Plan(), X.DeviceBuffer(), std::max(X.Height(), 1),
-                                         X.Width()),
-                  "Solve");
+    if (!FrontsAre1d(L.frontType)) throw LogicError("Invalid front type for 1D solve");
+    detail::solve_nodal(L, X.DeviceBuffer(), X.Height(), X.Width());
+}
+template <typename F>
+inline void Solve(const DistSymmInfo& info, const DistSymmFrontTree<F>& L,
+                  DistNodalMatrix<F>& X) {
+    (void)info;
+    if (FrontsAre1d(L.frontType)) throw LogicError("Invalid front type for 2D solve");
+    detail::solve_nodal(L, X.DeviceBuffer(), X.Height(), X.Width());
 }
 
 // FP64 iterative refinement of A X = B (no reference counterpart): X is

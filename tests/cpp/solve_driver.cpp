@@ -2,7 +2,7 @@
 // operator, random X, Y := A X, natural nested dissection, LDL, Pull, Solve,
 // Push, error norms), written against the cliq:: surface of clique.hpp.
 // Exits non-zero if the relative error or the backward error is too large.
-//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [real|complex] [ldl|block]
+//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [real|complex] [ldl|block] [1d|2d]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -57,12 +57,23 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
 
     DistMultiVec<F> Xc(N, numRhs, comm);
     double relErr = 0;
-    if (!fp32) {
+    bool wrongKindThrew = false;
+    if (!fp32 && FrontsAre1d(ftype)) {
         DistNodalMultiVec<F> YNodal;
         YNodal.Pull(inverseMap, info, Y);
         Solve(info, frontTree, YNodal);
         YNodal.Push(inverseMap, info, Xc);
+        DistNodalMatrix<F> M(inverseMap, info, Y);
+        try { Solve(info, frontTree, M); } catch (const LogicError&) { wrongKindThrew = true; }
+    } else if (!fp32) {
+        // 2D front types solve against a DistNodalMatrix (solve.hpp:67-92)
+        DistNodalMatrix<F> YNodal(inverseMap, info, Y);
+        Solve(info, frontTree, YNodal);
+        YNodal.Push(inverseMap, info, Xc);
+        DistNodalMultiVec<F> V(inverseMap, info, Y);
+        try { Solve(info, frontTree, V); } catch (const LogicError&) { wrongKindThrew = true; }
     } else {
+        wrongKindThrew = true;
         cliqg_refine_info r = Refine(frontTree, Y, Xc, 1e-10, 30);
         std::printf("refinement: %d iterations, backward error %.3e\n", r.iterations,
                     r.max_backward_error);
@@ -85,8 +96,9 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
         threw = true;
     }
     const double tol = fp32 ? 1e-8 : 1e-10;
-    if (relErr > tol || !threw) {
-        std::printf("FAILED (relative error %.3e, refactor guard %d)\n", relErr, (int)threw);
+    if (relErr > tol || !threw || !wrongKindThrew) {
+        std::printf("FAILED (relative error %.3e, refactor guard %d, 1D/2D guard %d)\n", relErr,
+                    (int)threw, (int)wrongKindThrew);
         return 1;
     }
     std::printf("PASSED\n");
@@ -101,7 +113,9 @@ int main(int argc, char* argv[]) {
     const int k = argc > 4 ? std::atoi(argv[4]) : 5;
     const bool fp32 = argc > 5 && std::string(argv[5]) == "fp32";
     const bool cpx = argc > 6 && std::string(argv[6]) == "complex";
-    const SymmFrontType ftype = argc > 7 && std::string(argv[7]) == "block" ? BLOCK_LDL_1D : LDL_1D;
+    const bool blk = argc > 7 && std::string(argv[7]) == "block";
+    const bool twoD = argc > 8 && std::string(argv[8]) == "2d";
+    const SymmFrontType ftype = blk ? (twoD ? BLOCK_LDL_2D : BLOCK_LDL_1D) : (twoD ? LDL_2D : LDL_1D);
     int rc = 1;
     try {
         rc = cpx ? run<std::complex<double>>(n1, n2, n3, k, fp32, ftype)

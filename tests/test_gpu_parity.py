@@ -335,3 +335,30 @@ def test_graph_replays_match_direct_launches(device, precision):
             g.solve(xg)
             d.solve(xd)
             assert np.array_equal(xg.cpu().numpy(), xd.cpu().numpy()), (scale, rep)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cut", [(10, 16), (16, 64)])
+def test_grf_diffusion_matches_oracle(device, n, cut):
+    """Config-4 family (7-point diffusion with a log-normal coefficient):
+    strongly varying but SPD; FP64 fronts and solutions against the oracle,
+    and FP32 factor + FP64 refinement to 1e-10."""
+    ip, ind, v, dims = problem("grf", n, seed=5)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, cut)
+    t, f = oracle_tree(ip, ind, v, dims, cut)
+    for s in range(sym.num_nodes):
+        Lg, dg = ft.front(s)
+        Lo, do = f.front(s)
+        assert np.abs(Lg - Lo).max(initial=0) <= 1e-12 * max(np.abs(Lo).max(initial=0), 1.0)
+        assert np.abs(dg - do).max(initial=0) <= 1e-12 * np.abs(do).max(initial=1.0)
+    b = np.random.default_rng(31).standard_normal((N, 8))
+    x = to_dev(b)
+    ft.solve(x)
+    xg = x.cpu().numpy()
+    assert rel_diff(xg, f.solve(b)) <= 1e-11
+    assert backward_error(ip, ind, v, xg, b).max() <= 1e-14
+    sym32, ft32 = gpu_tree(ip, ind, v, dims, cut, precision="f32")
+    xr, info = ft32.refine(to_dev(b), tol=1e-10, max_iters=30)
+    assert info["converged"]
+    assert backward_error(ip, ind, v, xr.cpu().numpy(), b).max() <= 1e-10
