@@ -63,6 +63,7 @@ __device__ __forceinline__ cdouble gload(const cdouble* p) {
     const d2v v = *(const __attribute__((address_space(1))) d2v*)(p);
     return cdouble{v.x, v.y};
 }
+
 __device__ __forceinline__ void gstore(float* p, float v) {
     *(__attribute__((address_space(1))) float*)(p) = v;
 }
@@ -240,8 +241,6 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
     else { kA = tid % TK; iA = tid / TK; dIA = NT / TK; dKA = 0; }
     if (transB) { jB = tid % TN; kB = tid / TN; dJB = 0; dKB = NT / TN; }
     else { kB = tid % TK; jB = tid / TK; dJB = NT / TK; dKB = 0; }
-    const int64_t offA = (int64_t)(transA ? dIA : dKA) * d.lda;
-    const int64_t offB = (int64_t)(transB ? dKB : dJB) * d.ldb;
     const int64_t stepA = transA ? TK : (int64_t)TK * d.lda;
     const int64_t stepB = transB ? (int64_t)TK * d.ldb : TK;
     unsigned rowOK = 0, colOK = 0;   // bit r: element r's row (column) is inside the matrix
@@ -253,6 +252,10 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
     const int ldsB = kB * (TN + LPAD) + jB, ldsBStep = dKB * (TN + LPAD) + dJB;
 
     T ra[LA], rb[LB];
+    // element r of a thread sits r*offA elements after its first one (one of
+    // dIA, dKA is zero); pA is the lane's pointer at the current chunk
+    const int64_t offA = (int64_t)(transA ? dIA : dKA) * d.lda;
+    const int64_t offB = (int64_t)(transB ? dKB : dJB) * d.ldb;
     const T* pA = nullptr;
     const T* pB = nullptr;
     auto start_at = [&](int k0) {
@@ -261,35 +264,72 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
         pB = transB ? d.B + (col0 + jB) + (int64_t)(k0 + kB) * d.ldb
                     : d.B + (k0 + kB) + (int64_t)(col0 + jB) * d.ldb;
     };
-    // Loads never branch: an element outside the matrix reads the operand's
-    // first entry and is replaced by zero when it is stored to LDS, so all of
-    // a chunk's loads are in flight together and the only wait is before the
-    // store (a divergent load would wait inside its branch).
+    // Loads never branch per element: an element outside the matrix reads the
+    // operand's first entry and is replaced by zero when it is stored to LDS,
+    // so all of a chunk's loads are in flight together and the only wait is
+    // before the store (a divergent load would wait inside its branch). Tiles
+    // inside the matrix take a uniform fast path with no masks at all: every
+    // VALU instruction here is issue time the MFMAs do not get.
     W rdw = one<W>();
     unsigned okA = 0, okB = 0;
-    bool okD = false;
+    bool okD = false, fastA = false, fastB = false;
+    const bool innerA = row0 + TM <= d.M, innerB = col0 + TN <= d.N;
     auto load = [&](int k0) {
         const bool fullK = k0 + TK <= d.K;
         if constexpr (SCALED) {
             okD = tid < TK && k0 + tid < d.K;
             rdw = gload(scaleK + (okD ? k0 + tid : 0));
         }
-        okA = 0;
-        okB = 0;
+        fastA = innerA && fullK;
+        fastB = innerB && fullK;
+        if (fastA) {
 #pragma unroll
-        for (int r = 0; r < LA; ++r) {
-            const bool ok = ((rowOK >> r) & 1u) && (fullK || k0 + kA + r * dKA < d.K);
-            okA |= ok ? 1u << r : 0u;
-            ra[r] = gload(ok ? pA + r * offA : d.A);
+            for (int r = 0; r < LA; ++r) ra[r] = gload(pA + r * offA);
+        } else {
+            okA = 0;
+#pragma unroll
+            for (int r = 0; r < LA; ++r) {
+                const bool ok = ((rowOK >> r) & 1u) && (fullK || k0 + kA + r * dKA < d.K);
+                okA |= ok ? 1u << r : 0u;
+                ra[r] = gload(ok ? pA + r * offA : d.A);
+            }
         }
+        if (fastB) {
 #pragma unroll
-        for (int r = 0; r < LB; ++r) {
-            const bool ok = ((colOK >> r) & 1u) && (fullK || k0 + kB + r * dKB < d.K);
-            okB |= ok ? 1u << r : 0u;
-            rb[r] = gload(ok ? pB + r * offB : d.B);
+            for (int r = 0; r < LB; ++r) rb[r] = gload(pB + r * offB);
+        } else {
+            okB = 0;
+#pragma unroll
+            for (int r = 0; r < LB; ++r) {
+                const bool ok = ((colOK >> r) & 1u) && (fullK || k0 + kB + r * dKB < d.K);
+                okB |= ok ? 1u << r : 0u;
+                rb[r] = gload(ok ? pB + r * offB : d.B);
+            }
         }
         pA += stepA;
         pB += stepB;
+    };
+    // stage n elements v[] at p0[off + r*step] (re) and p1[...] (im), with the
+    // mask and conjugation decided once per chunk (uniform branches)
+    auto stage = [&](const T* v, int n, unsigned ok, bool fast, bool conj, R* p0, R* p1, int off,
+                     int step) {
+        if (fast && !conj) {
+#pragma unroll
+            for (int r = 0; r < (LA > LB ? LA : LB); ++r)
+                if (r < n) {
+                    p0[off + r * step] = re(v[r]);
+                    if constexpr (CPX) p1[off + r * step] = im(v[r]);
+                }
+        } else {
+#pragma unroll
+            for (int r = 0; r < (LA > LB ? LA : LB); ++r)
+                if (r < n) {
+                    T x = (fast || ((ok >> r) & 1u)) ? v[r] : zero<T>();
+                    if (conj) x = conj_(x);
+                    p0[off + r * step] = re(x);
+                    if constexpr (CPX) p1[off + r * step] = im(x);
+                }
+        }
     };
     auto store = [&](int buf) {
         if constexpr (SCALED)
@@ -299,24 +339,8 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
                 As[buf][0][tid][TM] = re(dv);
                 if constexpr (CPX) As[buf][NP - 1][tid][TM] = im(dv);
             }
-        R* a0 = &As[buf][0][0][0];
-        R* a1 = &As[buf][NP - 1][0][0];
-        R* b0 = &Bs[buf][0][0][0];
-        R* b1 = &Bs[buf][NP - 1][0][0];
-#pragma unroll
-        for (int r = 0; r < LA; ++r) {
-            T v = ((okA >> r) & 1u) ? ra[r] : zero<T>();
-            if (conjA) v = conj_(v);
-            a0[ldsA + r * ldsAStep] = re(v);
-            if constexpr (CPX) a1[ldsA + r * ldsAStep] = im(v);
-        }
-#pragma unroll
-        for (int r = 0; r < LB; ++r) {
-            T v = ((okB >> r) & 1u) ? rb[r] : zero<T>();
-            if (conjB) v = conj_(v);
-            b0[ldsB + r * ldsBStep] = re(v);
-            if constexpr (CPX) b1[ldsB + r * ldsBStep] = im(v);
-        }
+        stage(ra, LA, okA, fastA, conjA, &As[buf][0][0][0], &As[buf][NP - 1][0][0], ldsA, ldsAStep);
+        stage(rb, LB, okB, fastB, conjB, &Bs[buf][0][0][0], &Bs[buf][NP - 1][0][0], ldsB, ldsBStep);
     };
 
     // K range of this row tile: triangular op(A) bounds it by the tile's rows
