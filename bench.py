@@ -345,7 +345,7 @@ def main():
         "fp64": fp64,
         "speedup_fp32ir_vs_fp64": (fp64["time_to_tol_ms"] / (t32 * 1e3)) if fp64 else None,
         "roofline": {"bound": "mfma",
-                     "kernel": "k_gemm<cfloat,64,64,2> (pivot-scaled Schur/panel updates, FP32 factor)",
+                     "kernel": "k_gemm<cfloat,64,64,3> (Schur, panel and outer-panel updates, FP32 factor)",
                      "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"], "traffic": traffic,
                      "traffic_source": traffic_src,

@@ -442,7 +442,13 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
                                   accS[a][b][r] - accR[a][b][r] - accI[a][b][r]);
                 else
                     v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
-                if (scaleN) v = cvt<T>(cvt<W>(v) / scaleN[gj]);
+                if (scaleN) {
+                    v = cvt<T>(cvt<W>(v) / scaleN[gj]);
+                    // W = L D from the rounded L, exactly as a pivot-scaled
+                    // update would form it, so updates from W stay consistent
+                    // with the stored factor
+                    if (d.C2) gstore(d.C2 + gi + (int64_t)gj * d.ldc, v * cvt<T>(scaleN[gj]));
+                }
                 if (neg) v = -v;
                 T* p = d.C + gi + (int64_t)gj * d.ldc;
                 if (accum) v = gload(p) + v;
@@ -463,6 +469,8 @@ static void launch_gemm_shape(const GemmDesc<T>* d, const int* ts, int nd, int n
         hipLaunchKernelGGL((k_gemm<T, TM, TN, 0, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
     else if (role == 2)
         hipLaunchKernelGGL((k_gemm<T, TM, TN, 2, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
+    else if (role == 3)
+        hipLaunchKernelGGL((k_gemm<T, TM, TN, 3, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
     else
         hipLaunchKernelGGL((k_gemm<T, TM, TN, 1, KT, WM, WN>), dim3(nt), block, 0, s, d, ts, nd);
 }

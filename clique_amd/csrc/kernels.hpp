@@ -46,6 +46,7 @@ struct GemmDesc {
     int64_t lda, ldb, ldc;
     int M, N, K;
     int flags;
+    T* C2 = nullptr;      // with scaleN: optional second output (ld ldc), C * diag(scaleN)
 };
 
 // Dense copy dst(0:rows, 0:cols) := src(0:rows, 0:cols), both column-major.
@@ -112,8 +113,9 @@ struct ZeroSeg {
 template <class T>
 void launch_zero_segments(const ZeroSeg<T>* d_segs, const int* d_chunk_start, int nseg,
                           int nchunks, hipStream_t s);
-// role: 0 factor, 1 solve, 2 factor with every descriptor scaled by scaleK
-// (separate instantiations keep the unscaled kernels' register budget).
+// role: 0 factor TRSM-like products, 1 solve, 2 factor with every descriptor
+// scaled by scaleK, 3 factor updates (separate instantiations keep the
+// unscaled kernels' register budget and name the launches in traces).
 template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
                  int variant, int role, hipStream_t s);

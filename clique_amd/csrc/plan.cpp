@@ -121,6 +121,7 @@ struct Step {
     double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
     int variant = 0;      // GEMM tile shape
     bool scaled = false;  // GEMM descriptors carry pivots (scaleK)
+    bool update = false;  // a factor update (profiled as gemm_update, role 3 kernels)
     int stream = 0;       // 0: the plan's stream, 1: its side stream
     int event = -1;       // S_RECORD / S_WAIT: index into the plan's dependency events
 };
@@ -197,8 +198,14 @@ struct StepBuilder {
     // triB: one operand is triangular (an inverted diagonal block, or a
     // triangular op(A) whose K range the kernel clamps), so only half of the
     // K products count as algorithmic work.
-    // variant >= 0 forces a tile shape (e.g. 128-row tiles for an in-place
-    // product whose rows must all sit in one tile).
+    // a factor update (named and instantiated apart from the TRSM-like products)
+    void update(const std::vector<GemmDesc<T>>& in) {
+        const size_t n0 = steps.size();
+        gemm(in);
+        if (steps.size() > n0) steps.back().update = true;
+    }
+
+    // variant >= 0 forces a tile shape.
     void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false, int forced = -1) {
         std::vector<GemmDesc<T>> ds;
         std::vector<int> start;
@@ -272,6 +279,9 @@ struct Numeric : NumericBase {
     using V = typename Scalar<T>::wide;
     Plan& P;
     DevBuf fpool, upool, dpool, linv, linvw, wtmp;
+    // W = L*D below the diagonal blocks of the current level's fronts (the
+    // TRSM writes it next to L), the left operand of every update
+    DevBuf wbuf;
     Arena farena, barena;
     std::vector<Step> fsteps, bsteps;   // LDL and block-LDL factor schedules
     int nextEvent = 0;                  // dependency events used by the schedules
@@ -316,6 +326,13 @@ struct Numeric : NumericBase {
                 mx = std::max(mx, a);
             }
             wtmp.alloc(std::max<int64_t>(mx, 1) * sizeof(T));
+            int64_t mw = 0;
+            for (const auto& lv : P.levels) {
+                int64_t a = 0;
+                for (int s : lv) a += (int64_t)P.fronts[s].m * P.fronts[s].n;
+                mw = std::max(mw, a);
+            }
+            wbuf.alloc(std::max<int64_t>(mw, 1) * sizeof(T));
         }
         build_factor_schedule(farena, fsteps, false);
         P.ensure_dep_events(nextEvent);
@@ -326,7 +343,7 @@ struct Numeric : NumericBase {
     size_t device_bytes() const override {
         size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
                    barena.dev.bytes + bsave.bytes + bsinv.bytes + btmp.bytes + rbuf.bytes +
-                   identity.bytes + linvw.bytes + wtmp.bytes;
+                   identity.bytes + linvw.bytes + wtmp.bytes + wbuf.bytes;
         for (auto& kv : solves)
             b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
                  kv.second->tmp.bytes;
@@ -426,6 +443,15 @@ struct Numeric : NumericBase {
             int maxn = 0;
             for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
             int pendingB2 = -1;   // side-stream update the next panel's B1 must wait for
+            std::map<int, int64_t> wOff;
+            {
+                int64_t a = 0;
+                for (int s : lv) {
+                    wOff[s] = a;
+                    a += (int64_t)P.fronts[s].m * P.fronts[s].n;
+                }
+            }
+            auto Wf = [&](int s) { return wbuf.as<T>() + wOff[s]; };
             // Two-level panels: 64-wide diagonal blocks whose updates stay
             // inside an OB-wide outer panel, then one rank-OB update of the
             // front's trailing columns (K = OB instead of K = 64 passes).
@@ -442,12 +468,17 @@ struct Numeric : NumericBase {
                         dd.push_back(DiagDesc<T>{base + k + (int64_t)k * f.m, D(s) + k, Li(s, k), f.m,
                                                  b, s, k});
                         T* a21 = base + (k + b) + (int64_t)k * f.m;
-                        if (f.m - k - b > 0)
-                            trsm.push_back(GemmDesc<T>{a21, Li(s, k), a21, nullptr, D(s) + k, f.m, NB,
-                                                       f.m, f.m - k - b, b, b, G_TRANSB | herm});
+                        T* w21 = Wf(s) + (k + b) + (int64_t)k * f.m;
+                        if (f.m - k - b > 0) {
+                            // L21 = W21 D^{-1} into the front, W21 = A21 Linv^T next to it
+                            GemmDesc<T> t{a21, Li(s, k), a21, nullptr, D(s) + k, f.m, NB, f.m,
+                                          f.m - k - b, b, b, G_TRANSB | herm};
+                            t.C2 = w21;
+                            trsm.push_back(t);
+                        }
                         if (c1 - k - b > 0) {
                             T* a22 = base + (k + b) + (int64_t)(k + b) * f.m;
-                            upd.push_back(GemmDesc<T>{a21, a21, a22, D(s) + k, nullptr, f.m, f.m, f.m,
+                            upd.push_back(GemmDesc<T>{w21, a21, a22, nullptr, nullptr, f.m, f.m, f.m,
                                                       f.m - k - b, c1 - k - b, b,
                                                       G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
                         }
@@ -459,7 +490,7 @@ struct Numeric : NumericBase {
                                              fl * (Scalar<T>::complex ? 4 : 1), 0.0});
                     }
                     sb.gemm(trsm, true);
-                    sb.gemm(upd);
+                    sb.update(upd);
                 }
                 // Outer update with a lookahead of one panel: the next outer
                 // panel's columns (B1) on the plan's stream, the columns after
@@ -473,12 +504,14 @@ struct Numeric : NumericBase {
                     if (f.n <= k0 || f.n - c1 <= 0) continue;
                     const int c2 = std::min(f.n, c1 + OB);
                     T* l1 = L(s) + c1 + (int64_t)k0 * f.m;
-                    b1.push_back(GemmDesc<T>{l1, l1, L(s) + c1 + (int64_t)c1 * f.m, D(s) + k0, nullptr,
+                    T* w1 = Wf(s) + c1 + (int64_t)k0 * f.m;
+                    b1.push_back(GemmDesc<T>{w1, l1, L(s) + c1 + (int64_t)c1 * f.m, nullptr, nullptr,
                                              f.m, f.m, f.m, f.m - c1, c2 - c1, c1 - k0,
                                              G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
                     if (f.n - c2 > 0) {
                         T* l2 = L(s) + c2 + (int64_t)k0 * f.m;
-                        b2.push_back(GemmDesc<T>{l2, l2, L(s) + c2 + (int64_t)c2 * f.m, D(s) + k0,
+                        T* w2 = Wf(s) + c2 + (int64_t)k0 * f.m;
+                        b2.push_back(GemmDesc<T>{w2, l2, L(s) + c2 + (int64_t)c2 * f.m, nullptr,
                                                  nullptr, f.m, f.m, f.m, f.m - c2, f.n - c2, c1 - k0,
                                                  G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
                     }
@@ -492,12 +525,12 @@ struct Numeric : NumericBase {
                     sb.record(evL);
                     sb.on(1);
                     sb.wait(evL);
-                    sb.gemm(b2);
+                    sb.update(b2);
                     sb.record(evB2);
                     sb.on(0);
                 }
                 if (pendingB2 >= 0) sb.wait(pendingB2);
-                sb.gemm(b1);
+                sb.update(b1);
                 pendingB2 = evB2;
             }
             if (pendingB2 >= 0) {
@@ -509,10 +542,11 @@ struct Numeric : NumericBase {
                 const FrontLayout& f = P.fronts[s];
                 if (f.u == 0 || f.n == 0) continue;
                 T* l21 = L(s) + f.n;
-                syrk.push_back(GemmDesc<T>{l21, l21, U(s), D(s), nullptr, f.m, f.m, f.u, f.u, f.u,
-                                           f.n, G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                syrk.push_back(GemmDesc<T>{Wf(s) + f.n, l21, U(s), nullptr, nullptr, f.m, f.m, f.u,
+                                           f.u, f.u, f.n,
+                                           G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
             }
-            sb.gemm(syrk);
+            sb.update(syrk);
             if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp);
             else wide_inverse(sb, lv);
         }
@@ -844,7 +878,7 @@ struct Numeric : NumericBase {
                     break;
                 case S_GEMM:
                     launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                   s.variant, s.scaled ? 2 : (S ? 1 : 0), st);
+                                   s.variant, s.scaled ? 2 : (s.update ? 3 : (S ? 1 : 0)), st);
                     break;
                 case S_GATHER:
                     launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
@@ -868,7 +902,7 @@ struct Numeric : NumericBase {
                 // factor GEMMs by instantiation: pivot-scaled updates (Schur,
                 // panel and outer-panel updates) apart from the TRSM-like rest
                 const char* name = (s.kind == S_GEMM && !S)
-                                       ? (s.scaled ? "gemm_update" : "gemm_trsm")
+                                       ? ((s.update || s.scaled) ? "gemm_update" : "gemm_trsm")
                                        : step_name(kind);
                 P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1});
             }
