@@ -1,6 +1,8 @@
 // Device kernels of libcliqg for gfx950 (CDNA4). See kernels.hpp.
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace cliqg {
 
 namespace {
@@ -180,7 +182,8 @@ int gemm_tiles(int M, int N, int flags, int variant) {
 // ---------------------------------------------------------------------------
 template <class T, int TM, int TN, int ROLE, int TK, int WM = 2, int WN = 2>
 __global__ __launch_bounds__(64 * WM * WN)
-__attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) void k_gemm(const GemmDesc<T>* __restrict__ descs,
+__attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
+                                   : (sizeof(T) <= 8 ? 4 : 2), 8))) void k_gemm(const GemmDesc<T>* __restrict__ descs,
                                                        const int* __restrict__ tile_start,
                                                        int ndesc) {
     using R = typename Scalar<T>::real;
@@ -331,7 +334,8 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
                 }
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](auto bufc) {
+        constexpr int buf = decltype(bufc)::value;
         if constexpr (SCALED)
             if (tid < TK) {
                 T dv = zero<T>();
@@ -351,11 +355,13 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
     start_at(kbeg);
     if (nk > 0) {
         load(kbeg);
-        store(0);
+        store(std::integral_constant<int, 0>{});
         __syncthreads();
     }
-    for (int c = 0; c < nk; ++c) {
-        const int buf = c & 1;
+    // One K chunk from LDS buffer BUF (a compile-time constant: the loop below
+    // runs chunks in pairs, so LDS addresses fold into instruction offsets).
+    auto chunk = [&](int c, auto bufc) {
+        constexpr int buf = decltype(bufc)::value;
         if (c + 1 < nk) load(kbeg + (c + 1) * TK);
         // raised wave priority while feeding the MFMAs (measured: +3 % for
         // complex FP64, neutral elsewhere)
@@ -419,8 +425,12 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2 : 1, 8))) v
             }
         }
         __builtin_amdgcn_s_setprio(0);
-        if (c + 1 < nk) store(buf ^ 1);
+        if (c + 1 < nk) store(std::integral_constant<int, buf ^ 1>{});
         __syncthreads();
+    };
+    for (int c = 0; c < nk; c += 2) {
+        chunk(c, std::integral_constant<int, 0>{});
+        if (c + 1 < nk) chunk(c + 1, std::integral_constant<int, 1>{});
     }
 
     const W* scaleN = reinterpret_cast<const W*>(d.scaleN);
