@@ -134,21 +134,29 @@ def measured_traffic(cfg, n):
 
 
 def factor_only(args):
-    """Two FP32 factorizations and a 1 GiB device copy, nothing else: the
-    program the byte counters are collected on (the copy calibrates them)."""
+    """Two profiled FP32 factorizations (launches serialized on one stream, as
+    in the profiled factorization of the benchmark line) and a 1 GiB device
+    copy, nothing else: the program the kernel statistics and byte counters
+    are collected on (the copy calibrates the counters)."""
     import torch
     import clique_amd as cq
     ip, ind, v, dims, b, cplx, desc = workload(args.config, args.n)
     sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
     vals = torch.tensor(v, device="cuda:0")
     p = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+    p.set_profiling(True)
     for _ in range(2):
         p.factor(vals, precision="f32")
+    prof = p.profile()
     src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"factor_ms": p.last_factor_ms(), "copy_bytes": src.numel() * 4}))
+    g = prof["gemm_update"]
+    print(json.dumps({"factor_ms": p.last_factor_ms(), "copy_bytes": src.numel() * 4,
+                      "gemm_update_launches": g["launches"],
+                      "gemm_update_avg_ms": g["ms"] / g["launches"],
+                      "gemm_update_tflops": g["flops"] / (g["ms"] * 1e-3) / 1e12}))
 
 
 def main():

@@ -517,7 +517,9 @@ struct Numeric : NumericBase {
                     }
                 }
                 int evB2 = -1;
-                if (!b2.empty()) {
+                if (!b2.empty() && !P.lookahead) {
+                    sb.update(b2);
+                } else if (!b2.empty()) {
                     // B2 needs this panel's final L columns, and runs after
                     // the previous B2 on the side stream
                     const int evL = nextEvent++;
@@ -849,13 +851,15 @@ struct Numeric : NumericBase {
 
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
         for (const Step& s : steps) {
-            hipStream_t st = s.stream ? P.stream2 : P.stream;
-            if (s.kind == S_RECORD) {
-                CQ_HIP(hipEventRecord(P.depEvents[s.event], st));
-                continue;
-            }
-            if (s.kind == S_WAIT) {
-                CQ_HIP(hipStreamWaitEvent(st, P.depEvents[s.event], 0));
+            // profiling serializes the side stream onto the main one, so that
+            // every launch's event interval is its own kernel time
+            hipStream_t st = (s.stream && !P.profiling) ? P.stream2 : P.stream;
+            if (s.kind == S_RECORD || s.kind == S_WAIT) {
+                if (P.profiling) continue;
+                if (s.kind == S_RECORD)
+                    CQ_HIP(hipEventRecord(P.depEvents[s.event], st));
+                else
+                    CQ_HIP(hipStreamWaitEvent(st, P.depEvents[s.event], 0));
                 continue;
             }
             hipEvent_t e0 = nullptr;
@@ -1029,6 +1033,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     : sym(std::move(symIn)), complex(cpx), conjugate(conj), stream(st) {
     const Symbolic& S = *sym;
     N = S.N;
+    if (const char* e = std::getenv("CLIQG_LOOKAHEAD")) lookahead = std::atoi(e) != 0;
     if (const char* e = std::getenv("CLIQG_SOLVE_BLOCK")) {
         const int v = std::atoi(e);
         if (v < NB || v % NB) throw Error(1, "CLIQG_SOLVE_BLOCK must be a positive multiple of 64");

@@ -80,6 +80,7 @@ public:
     std::vector<std::vector<int>> levels;   // fronts of each height
     int64_t factorEntries = 0, updateEntries = 0, linvEntries = 0, linvWEntries = 0, wRows = 0;
     int sb = SB_DEFAULT;   // solve block width (CLIQG_SOLVE_BLOCK overrides; multiple of 64)
+    bool lookahead = true; // side-stream outer-panel updates (CLIQG_LOOKAHEAD=0 disables)
     std::vector<int64_t> hostRowOff;
     hipStream_t stream = nullptr;
     bool ownStream = false;
