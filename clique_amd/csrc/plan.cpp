@@ -851,11 +851,10 @@ struct Numeric : NumericBase {
 
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
         for (const Step& s : steps) {
-            // profiling serializes the side stream onto the main one, so that
-            // every launch's event interval is its own kernel time
-            hipStream_t st = (s.stream && !P.profiling) ? P.stream2 : P.stream;
+            // a launch's profiling events go on its own stream: the interval is
+            // the kernel's duration as it runs beside the other stream's work
+            hipStream_t st = s.stream ? P.stream2 : P.stream;
             if (s.kind == S_RECORD || s.kind == S_WAIT) {
-                if (P.profiling) continue;
                 if (s.kind == S_RECORD)
                     CQ_HIP(hipEventRecord(P.depEvents[s.event], st));
                 else
@@ -907,7 +906,8 @@ struct Numeric : NumericBase {
                 // panel and outer-panel updates) apart from the TRSM-like rest
                 const char* name = (s.kind == S_GEMM && !S)
                                        ? ((s.update || s.scaled) ? "gemm_update" : "gemm_trsm")
-                                       : step_name(kind);
+                                   : (s.kind == S_ZERO && S) ? "solve_zero"
+                                                             : step_name(kind);
                 P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1});
             }
         }
