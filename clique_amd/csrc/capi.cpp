@@ -302,7 +302,7 @@ cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* 
         }
         if (!conv && k > 0)
             throw cliqg::Error(CLIQG_ERR_NONCONVERGED,
-                               "refinement stopped at backward error " + std::to_string(eta));
+                               "refinement stopped at backward error " + cliqg::sci(eta));
     });
 }
 

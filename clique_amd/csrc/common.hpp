@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -126,6 +127,13 @@ struct Error : std::runtime_error {
     int code;
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
+
+// a number in scientific notation for messages
+inline std::string sci(double v) {
+    char b[32];
+    std::snprintf(b, sizeof(b), "%.3e", v);
+    return b;
+}
 
 #define CQ_HIP(call)                                                                    \
     do {                                                                                \

@@ -1375,7 +1375,7 @@ void Plan::adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, i
         double eta = 0.0;
         refine(d_g, ldg, d_lambda, ldl, k, 1e-12, 30, &it, &conv, &eta);
         if (!conv)
-            throw Error(3, "adjoint refinement stopped at backward error " + std::to_string(eta));
+            throw Error(3, "adjoint refinement stopped at backward error " + sci(eta));
     }
     dispatch(complex, precision, [&](auto* tag) {
         using T = std::remove_pointer_t<decltype(tag)>;
