@@ -157,8 +157,8 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 
 }  // namespace
 
-int gemm_tile_m(int variant) { return variant == 0 ? 64 : 128; }
-int gemm_tile_n(int variant) { return (variant >= 2) ? 128 : 64; }
+int gemm_tile_m(int variant) { return variant == 5 ? 32 : (variant == 0 ? 64 : 128); }
+int gemm_tile_n(int variant) { return variant == 5 ? 32 : ((variant >= 2) ? 128 : 64); }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
@@ -506,6 +506,9 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
             launch_gemm_shape<T, 128, 128, 16, 2, 2>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
             throw Error(1, "128x128 GEMM tiles are not built for complex double");
+    } else if (variant == 5) {
+        launch_gemm_shape<T, 32, 32>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+
     } else {
         launch_gemm_shape<T, 64, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     }
@@ -615,20 +618,31 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 // ---------------------------------------------------------------------------
 // Diagonal block: unpivoted LDL^T of a b x b block (b <= NB) in FP64 (field
 // type) and the inverse of its unit lower factor. One workgroup of 256
-// threads per front; thread (ti, tj) of a 16 x 16 grid keeps the 4 x 4 tile
-// M[4ti.., 4tj..] of a 64 x 64 register-resident matrix whose lower triangle
+// threads per front. Thread (ti, tj) = (tid % 16, tid / 16) keeps the
+// elements M[ti + 16 r][tj + 16 q], r, q < 4, of a 64 x 64 register-resident
+// matrix (a cyclic layout: the columns already finished at step j are whole
+// slots q with 16 q + 15 <= j, which every wave skips). The lower triangle
 // holds the Schur-updated block (its column j is the unscaled L column at
-// step j) and whose strict upper triangle holds the transpose of the inverse
-// being built, li(q, i) at M[i][q]. Step j broadcasts column j of M through
-// LDS (double buffered, one barrier per step):
-//     a(i,c)  -= a(i,j) op(a(c,j)) / d_j          (lower, c > j)
-//     li(q,i) -= (a(q,j) / d_j) li(j,i)           (upper, i <= j < q; li(j,j) = 1)
+// step j), the strict upper triangle the transpose of the inverse being
+// built, li(c, i) at M[i][c]. Step j broadcasts column j of M through LDS
+// (double buffered, one barrier per step):
+//     a(i,c)  -= (a(i,j) / d_j) op(a(c,j))      lower, i >= c > j
+//     li(c,i) -= u_i (a(c,j) / d_j)             upper, i <= j < c,
+// with u_i = li(j,i) = M[i][j] for i < j and u_j = 1. Slots r > q hold only
+// lower elements and r < q only upper ones; the slot r == q is lower in the
+// threads with ti >= tj.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ double pivot_inverse(double a) { return 1.0 / a; }
+__device__ __forceinline__ cdouble pivot_inverse(cdouble a) {
+    const double s = 1.0 / (a.x * a.x + a.y * a.y);
+    return {a.x * s, -a.y * s};
+}
+
 // (Two waves per SIMD are plenty for this latency-bound kernel; the complex
 // instantiations need the registers.)
-template <class T>
+template <class T, bool HERM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_diag_ldl(
-    const DiagDesc<T>* __restrict__ descs, bool herm, int* status) {
+    const DiagDesc<T>* __restrict__ descs, int* status) {
     using W = typename Scalar<T>::wide;
     __shared__ W col[2][NB];
     __shared__ W dinv[NB];
@@ -636,7 +650,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
 
     const DiagDesc<T> d = descs[blockIdx.x];
     const int b = d.b, tid = threadIdx.x;
-    const int ti = tid >> 4, tj = tid & 15;
+    const int ti = tid & 15, tj = tid >> 4;
     W M[4][4];
     // all sixteen loads in flight at once (an element outside the lower
     // triangle reads A11(0,0) and is replaced by zero)
@@ -645,7 +659,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 4 * ti + r, c = 4 * tj + q;
+            const int i = ti + 16 * r, c = tj + 16 * q;
             const bool in = i < b && c < b && i >= c;
             raw[r][q] = gload(d.A11 + (in ? i + (int64_t)c * d.lda : 0));
         }
@@ -653,51 +667,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 4 * ti + r, c = 4 * tj + q;
+            const int i = ti + 16 * r, c = tj + 16 * q;
             M[r][q] = (i < b && c < b && i >= c) ? cvt<W>(raw[r][q]) : W{};
         }
+    const bool lowdiag = ti >= tj;
     W* Dout = reinterpret_cast<W*>(d.D);
     for (int j = 0; j < b; ++j) {
-        const int buf = j & 1;
-        if (tj == (j >> 2)) {
+        const int buf = j & 1, J = j >> 4;
+        if (tj == (j & 15)) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                if (q == (j & 3)) {
+                if (q == J) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) col[buf][4 * ti + r] = M[r][q];
+                    for (int r = 0; r < 4; ++r) col[buf][ti + 16 * r] = M[r][q];
                 }
         }
         __syncthreads();
         W piv = col[buf][j];
-        if (herm) piv = real_part_only(piv);
-        const W ip = one<W>() / piv;
+        if (HERM) piv = real_part_only(piv);
+        const W ip = pivot_inverse(piv);
         // pivots stay in LDS during the loop: a global store here would make
         // every barrier wait for it (the barrier's fence covers global memory)
         if (tid == 0) {
             dval[j] = piv;
             dinv[j] = ip;
         }
-        W x[4], cc[4];
+        // row multipliers: x_i = a(i,j)/d_j (lower), u_i (upper, zero for i > j)
+        W x[4], u[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            x[r] = col[buf][4 * ti + r] * ip;
-            cc[r] = col[buf][4 * tj + r];
+            const int i = ti + 16 * r;
+            const W a = col[buf][i];
+            x[r] = a * ip;
+            u[r] = (i < j) ? a : ((i == j) ? one<W>() : W{});
         }
-        // Branch-free update: lower (i >= c > j): M -= (a_ij / d_j) op(a_cj);
-        // upper (i <= j < c): M -= a_cj / d_j * l_ij with l_jj = 1. Every
-        // thread evaluates one product per element and keeps it by select
-        // (divergent branches per element cost far more than the product).
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int q = 0; q < 4; ++q) {
+            if (16 * q + 15 <= j) continue;  // every column of this slot is finished
+            const int c = tj + 16 * q;
+            const bool cact = c > j;
+            const W cv = col[buf][c];
+            const W g = cact ? (HERM ? conj_(cv) : cv) : W{};
+            const W xx = cact ? cv * ip : W{};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = 4 * ti + r, c = 4 * tj + q;
-                const bool lowerEl = i >= c;
-                const bool act = c > j && (lowerEl || i <= j);
-                const W f = pick(!lowerEl && i == j, ip, x[r]);
-                const W g = pick(lowerEl && herm, conj_(cc[q]), cc[q]);
-                M[r][q] = pick(act, M[r][q] - f * g, M[r][q]);
+            for (int r = 0; r < 4; ++r) {
+                if (r > q) {
+                    M[r][q] -= x[r] * g;
+                } else if (r < q) {
+                    if (16 * r <= j) M[r][q] -= u[r] * xx;
+                } else {
+                    M[r][q] -= pick(lowdiag, x[r], u[r]) * pick(lowdiag, g, xx);
+                }
             }
+        }
     }
     __syncthreads();
     for (int j = tid; j < b; j += blockDim.x) {
@@ -708,7 +730,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     for (int r = 0; r < 4; ++r)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int i = 4 * ti + r, c = 4 * tj + q;
+            const int i = ti + 16 * r, c = tj + 16 * q;
             // L11 (unit diagonal) back into the front
             if (i < b && c < b && i >= c)
                 d.A11[i + (int64_t)c * d.lda] = (i == c) ? one<T>() : cvt<T>(M[r][q] * dinv[c]);
@@ -728,7 +750,10 @@ template <class T>
 void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
                      hipStream_t s) {
     if (ndesc <= 0) return;
-    hipLaunchKernelGGL(k_diag_ldl<T>, dim3(ndesc), dim3(256), 0, s, d_descs, herm, d_status);
+    if (herm)
+        hipLaunchKernelGGL((k_diag_ldl<T, true>), dim3(ndesc), dim3(256), 0, s, d_descs, d_status);
+    else
+        hipLaunchKernelGGL((k_diag_ldl<T, false>), dim3(ndesc), dim3(256), 0, s, d_descs, d_status);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,12 +12,13 @@ namespace cliqg {
 constexpr int NB = 64;
 // Tile shapes of the grouped GEMM: variant 0 = 64x64 (4 waves), 1 = 128x64
 // (4 waves), 2 = 128x128 (8 waves), 3 = 128x128 (16 waves), 4 = 128x128
-// (4 waves of 64x64); 2-4 are not built for complex double (LDS).
+// (4 waves of 64x64), 5 = 32x32 (4 waves of 16x16, for latency-bound
+// launches); 2-4 are not built for complex double (LDS).
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
 template <class T>
 constexpr bool gemm_variant_allowed(int variant) {
-    return variant < 2 || !(Scalar<T>::complex && sizeof(T) == 16);
+    return variant < 2 || variant == 5 || !(Scalar<T>::complex && sizeof(T) == 16);
 }
 
 enum GemmFlags : int {

@@ -212,6 +212,9 @@ struct StepBuilder {
         int total = 0;
         double flops = 0.0, bytes = 0.0;
         const int variant = forced >= 0 ? forced : choose_variant(in);
+        for (const auto& d : in)
+            if ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(variant))
+                throw Error(1, "an in-place GEMM needs all of its columns in one tile");
         for (const auto& d : in) {
             const int t = gemm_tiles(d.M, d.N, d.flags, variant);
             if (t == 0) continue;
@@ -237,6 +240,8 @@ struct StepBuilder {
         steps.push_back(s);
     }
 
+    static constexpr double SMALL_TILE_LIMIT = 384.0;
+
     // Tile shape of a grouped launch: 64x64 tiles (four waves of 32x32) or
     // 128x128 tiles (four waves of 64x64, two blocks per CU). Cost = rounds of
     // 256 CUs the launch needs x MFMA work of one tile x the measured relative
@@ -245,6 +250,28 @@ struct StepBuilder {
     // three-multiplication products beat the large tile's four; complex FP64
     // has no large tile (LDS).
     static int choose_variant(const std::vector<GemmDesc<T>>& in) {
+        // Launches that fill at most about one round of 64x64 tiles are
+        // latency-bound: 32x32 tiles put four times as many workgroups on
+        // the CUs, each with a quarter of the MFMA chain (measured on complex
+        // FP32: 128x64 with K = 128 8 us instead of 15, 14400x64 with
+        // K = 128 16 instead of 20, 20000x64 with K = 7200 0.69 ms instead
+        // of 1.12).
+        static const bool smallTiles = [] {
+            const char* e = std::getenv("CLIQG_SMALL_TILES");
+            return !e || std::atoi(e) != 0;
+        }();
+        if (smallTiles) {
+            // An in-place product (C is A or B, the TRSM of the factor) needs
+            // every column of a row block in one tile: a second tile column
+            // would read A after the first one has overwritten it.
+            double tiles0 = 0.0;
+            bool inplace = false;
+            for (const auto& d : in) {
+                tiles0 += gemm_tiles(d.M, d.N, d.flags, 0);
+                inplace = inplace || ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(5));
+            }
+            if (!inplace && tiles0 > 0.0 && tiles0 <= SMALL_TILE_LIMIT) return 5;
+        }
         if (!gemm_variant_allowed<T>(4) || Scalar<T>::complex) return 0;
         const double rel4 = sizeof(T) == 4 ? 0.95 : 0.83;
         double cost[2] = {0.0, 0.0};
@@ -908,7 +935,8 @@ struct Numeric : NumericBase {
                                        ? ((s.update || s.scaled) ? "gemm_update" : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
                                                              : step_name(kind);
-                P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1});
+                P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1, s.ndesc, s.nwork,
+                                                  s.variant, s.stream ? 1 : 0});
             }
         }
         CQ_HIP(hipGetLastError());
@@ -1201,15 +1229,22 @@ hipEvent_t Plan::next_event() {
 }
 
 void Plan::resolve_profile() {
+    // CLIQG_PROFILE_TRACE=<file>: one line per profiled launch, appended
+    static const char* tracePath = std::getenv("CLIQG_PROFILE_TRACE");
+    FILE* trace = tracePath ? std::fopen(tracePath, "a") : nullptr;
     for (const Pending& p : pending) {
         float ms = 0.f;
         CQ_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+        if (trace)
+            std::fprintf(trace, "%s %.4f %.6e %.6e %d %d %d %d\n", p.name, ms, p.flops, p.bytes,
+                         p.ndesc, p.nwork, p.variant, p.stream);
         ProfileEntry& e = profile[p.name];
         e.launches += 1;
         e.ms += ms;
         e.flops += p.flops;
         e.bytes += p.bytes;
     }
+    if (trace) std::fclose(trace);
     pending.clear();
     evNext_ = 0;
 }

@@ -103,6 +103,7 @@ public:
         const char* name;
         double flops, bytes;
         hipEvent_t e0, e1;
+        int ndesc, nwork, variant, stream;
     };
     struct ProfileEntry {
         int launches = 0;

@@ -101,6 +101,23 @@ def test_complex_symmetric_helmholtz_fp64(device, n, cut):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-13), ("f32", 1e-5)])
+def test_pml40_factor_backward_error(device, precision, tol):
+    # fronts up to 2,360 x 760, whose TRSMs (C = A Linv^T in place) span many
+    # row tiles: the in-place product must keep a row block's columns in one
+    # tile (two 32-wide tile columns raced here, backward error 1e-5 in FP64)
+    ip, ind, v, dims = problem("pml", 40, pml=5)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 128, is_complex=True, precision=precision)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+    x = to_dev(b)
+    ft.solve(x)
+    eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
+    assert eta.max() <= tol, eta
+
+
+@pytest.mark.gpu
 def test_hermitian_ldlh_fp64(device):
     ip, ind, v, dims = hermitian_matrix(6)
     N = ip.size - 1
