@@ -562,6 +562,37 @@ void launch_copy2d(const CopyDesc<T>* d_descs, const int* d_chunk_start, int nde
     hipLaunchKernelGGL(k_copy2d<T>, dim3(nchunks), dim3(256), 0, s, d_descs, d_chunk_start, ndesc);
 }
 
+// One 1024-element chunk of one split product per workgroup; each element
+// sums its slices in order, so results do not depend on scheduling.
+template <class T>
+__global__ __launch_bounds__(256) void k_reduce_slices(const ReduceDesc<T>* __restrict__ descs,
+                                                       const int* __restrict__ chunk_start,
+                                                       int ndesc) {
+    const int g = blockIdx.x;
+    const int di = find_segment(chunk_start, ndesc, g);
+    const ReduceDesc<T> d = descs[di];
+    const int64_t total = (int64_t)d.rows * d.cols;
+    const int64_t base = (int64_t)(g - chunk_start[di]) * 1024;
+    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
+        const int64_t e = base + i;
+        if (e >= total) break;
+        T v = gload(d.ws + e);
+        for (int sl = 1; sl < d.slices; ++sl) v = v + gload(d.ws + sl * total + e);
+        if (d.flags & G_NEG) v = -v;
+        T* p = d.C + e % d.rows + (e / d.rows) * d.ldc;
+        if (d.flags & G_ACCUM) v = gload(p) + v;
+        gstore(p, v);
+    }
+}
+
+template <class T>
+void launch_reduce_slices(const ReduceDesc<T>* d_descs, const int* d_chunk_start, int ndesc,
+                          int nchunks, hipStream_t s) {
+    if (nchunks <= 0 || ndesc <= 0) return;
+    hipLaunchKernelGGL(k_reduce_slices<T>, dim3(nchunks), dim3(256), 0, s, d_descs, d_chunk_start,
+                       ndesc);
+}
+
 // ---------------------------------------------------------------------------
 // Assembly of A into the fronts (scatter through a precomputed map)
 // ---------------------------------------------------------------------------
@@ -1035,6 +1066,7 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
 #define CQ_INST_T(T)                                                                           \
     template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
     template void launch_copy2d<T>(const CopyDesc<T>*, const int*, int, int, hipStream_t);       \
+    template void launch_reduce_slices<T>(const ReduceDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \

@@ -59,6 +59,17 @@ struct CopyDesc {
     int rows, cols;
 };
 
+// Sum of the K slices of a split product: C = [C +] (-)sum_s W_s, summed in
+// slice order (slice s at ws + s*rows*cols, ld rows); flags G_ACCUM, G_NEG.
+template <class T>
+struct ReduceDesc {
+    const T* ws;
+    T* C;
+    int64_t ldc;
+    int rows, cols, slices;
+    int flags;
+};
+
 // Child-to-parent extend-add of one child's update matrix
 // (reference ldl/local.hpp:60-102).
 template <class T>
@@ -124,6 +135,9 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
 template <class T>
 void launch_copy2d(const CopyDesc<T>* d_descs, const int* d_chunk_start, int ndesc, int nchunks,
                    hipStream_t s);
+template <class T>
+void launch_reduce_slices(const ReduceDesc<T>* d_descs, const int* d_chunk_start, int ndesc,
+                          int nchunks, hipStream_t s);
 // number of output tiles of a descriptor (host side, must match the kernel)
 int gemm_tiles(int M, int N, int flags, int variant);
 

@@ -110,7 +110,7 @@ struct IntervalAlloc {
 };
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
-                S_RECORD, S_WAIT };
+                S_RECORD, S_WAIT, S_REDUCE };
 
 struct Step {
     int kind;
@@ -136,6 +136,7 @@ const char* step_name(int kind) {
         case 5: return "solve_extend_add";
         case 6: return "zero_update";
         case 8: return "copy";
+        case 11: return "solve_reduce";
         default: return "other";
     }
 }
@@ -191,6 +192,22 @@ struct StepBuilder {
         }
         if (keep.empty()) return;
         steps.push_back(Step{S_COPY, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
+                             false, 0.0, by});
+        steps.back().stream = cur;
+    }
+
+    void reduce(const std::vector<ReduceDesc<T>>& rs) {
+        std::vector<int> start;
+        int chunks = 0;
+        double by = 0.0;
+        for (const auto& r : rs) {
+            const int64_t e = (int64_t)r.rows * r.cols;
+            start.push_back(chunks);
+            chunks += (int)((e + 1023) / 1024);
+            by += (r.slices + ((r.flags & G_ACCUM) ? 2.0 : 1.0)) * e * sizeof(T);
+        }
+        if (rs.empty()) return;
+        steps.push_back(Step{S_REDUCE, arena.add(rs), arena.add(start), (int)rs.size(), chunks,
                              false, 0.0, by});
         steps.back().stream = cur;
     }
@@ -331,6 +348,7 @@ struct Numeric : NumericBase {
         Arena arena;
         std::vector<Step> fwd, bwd;
         DevBuf wpool, xn, xn2, tmp;
+        DevBuf splitWs;   // K slices of the split update-row products
         Graph graph;   // forward, diagonal and backward sweeps
     };
     Graph fgraph[2];   // LDL and block-LDL factorizations
@@ -680,6 +698,59 @@ struct Numeric : NumericBase {
         sb.copy(restore);
     }
 
+    // Split K of tall, skinny products with a long K (the update-row products
+    // of the solves, M x k with K = u or n): a launch with fewer than
+    // SPLIT_TILES 64x64 tiles leaves most CUs idle while each tile walks K in
+    // 16-deep steps, so every plain descriptor with K >= 2 SPLIT_MIN_K is cut
+    // into up to SPLIT_TARGET / tiles slices, written to ws and summed in
+    // slice order by a second launch. Returns the workspace entries used;
+    // with sb == nullptr nothing is scheduled.
+    static constexpr double SPLIT_TILES = 512.0, SPLIT_TARGET = 1024.0;
+    static constexpr int SPLIT_MIN_K = 512;
+    static int64_t split_bound() { return (int64_t)4096 * (int64_t)(SPLIT_TARGET + SPLIT_TILES); }
+
+    int64_t split_gemm(StepBuilder<T>* sb, const std::vector<GemmDesc<T>>& g, T* ws) {
+        double tiles = 0.0;
+        for (const auto& d : g) tiles += gemm_tiles(d.M, d.N, d.flags, 0);
+        const int want = (tiles > 0.0 && tiles < SPLIT_TILES) ? (int)std::ceil(SPLIT_TARGET / tiles)
+                                                              : 1;
+        std::vector<GemmDesc<T>> parts;
+        std::vector<ReduceDesc<T>> red;
+        int64_t off = 0;
+        for (const auto& d : g) {
+            const bool plain = !(d.flags & (G_KLO | G_KHI | G_LOWER | G_SYMM | G_INVK)) &&
+                               !d.scaleK && !d.scaleN && !d.C2 && d.C != d.A && d.C != d.B;
+            int S = 1;
+            if (want > 1 && plain && d.K >= 2 * SPLIT_MIN_K) S = std::min(want, d.K / SPLIT_MIN_K);
+            if (S <= 1) {
+                parts.push_back(d);
+                continue;
+            }
+            const int kc = (d.K + S - 1) / S;
+            S = (d.K + kc - 1) / kc;
+            for (int sl = 0; sl < S; ++sl) {
+                const int k0 = sl * kc;
+                GemmDesc<T> p = d;
+                p.A = (d.flags & G_TRANSA) ? d.A + k0 : d.A + (int64_t)k0 * d.lda;
+                p.B = (d.flags & G_TRANSB) ? d.B + (int64_t)k0 * d.ldb : d.B + k0;
+                p.C = ws + off + (int64_t)sl * d.M * d.N;
+                p.ldc = d.M;
+                p.K = std::min(kc, d.K - k0);
+                p.flags = d.flags & (G_TRANSA | G_TRANSB | G_CONJA | G_CONJB);
+                parts.push_back(p);
+            }
+            red.push_back(ReduceDesc<T>{ws + off, d.C, d.ldc, d.M, d.N, S,
+                                        d.flags & (G_ACCUM | G_NEG)});
+            off += (int64_t)S * d.M * d.N;
+        }
+        if (off > split_bound()) throw Error(5, "split-K workspace smaller than a launch needs");
+        if (sb) {
+            sb->gemm(parts);
+            sb->reduce(red);
+        }
+        return off;
+    }
+
     // Solve schedule for k right-hand sides. The front's own rows are the
     // nodal right-hand sides X (ld N); W_s (u x k, ld u) holds its update rows.
     // Forward, per level: clear W, extend-add child slot 0 then 1, then per
@@ -701,6 +772,8 @@ struct Numeric : NumericBase {
         // LDL sweeps ping-pong between xn and xn2: the forward sweep takes its
         // input in xn and leaves X in xn2, the backward sweep the reverse
         if (!blk) S.xn2.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
+        S.splitWs.alloc(split_bound() * sizeof(T));
+        T* ws = S.splitWs.template as<T>();
         // per-front n x k workspace of the block solves
         std::vector<int64_t> tmpOff(P.numFronts, 0);
         {
@@ -778,7 +851,7 @@ struct Numeric : NumericBase {
                                                     N, f.n, k, f.n, 0});
                         }
                         sb.copy(cs);
-                        sb.gemm(g);
+                        split_gemm(&sb, g, ws);
                     }
                     // SB-wide panels: X2_p := inv(L_pp) X_p (into the second
                     // buffer), X(below) -= L(below, p) X2_p
@@ -808,7 +881,7 @@ struct Numeric : NumericBase {
                         g.push_back(GemmDesc<T>{L(s) + f.n, X2(s), Wp(s), nullptr, nullptr, f.m, N,
                                                 f.u, f.u, k, f.n, G_ACCUM | G_NEG});
                     }
-                    sb.gemm(g);
+                    split_gemm(&sb, g, ws);
                 } else {
                     std::vector<SolveSeg> segs;
                     std::vector<int> rs;
@@ -840,11 +913,11 @@ struct Numeric : NumericBase {
                             xs.push_back(GemmDesc<T>{L(s), Tm(s), Xs(s), nullptr, nullptr, f.m, f.n,
                                                      N, f.n, k, f.n, G_ACCUM | G_NEG});
                         }
-                        sb.gemm(y);
-                        sb.gemm(xs);
+                        split_gemm(&sb, y, ws);
+                        split_gemm(&sb, xs, ws);
                         continue;
                     }
-                    sb.gemm(g);
+                    split_gemm(&sb, g, ws);
                     // SB-wide panels from the last: X_p := inv(L_pp)^T X2_p (^H if
                     // Hermitian; back into xn), X2(0:w0) -= L(p, 0:w0)^T X_p
                     const int maxw = (maxn + SB - 1) / SB;
@@ -914,6 +987,10 @@ struct Numeric : NumericBase {
                     launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
                                      S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k,
                                      P.dLow.as<int>(), st);
+                    break;
+                case S_REDUCE:
+                    launch_reduce_slices<T>(a.at<ReduceDesc<T>>(s.desc), a.at<int>(s.start),
+                                            s.ndesc, s.nwork, st);
                     break;
                 case S_COPY:
                     launch_copy2d<T>(a.at<CopyDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,

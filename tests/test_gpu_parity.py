@@ -101,14 +101,17 @@ def test_complex_symmetric_helmholtz_fp64(device, n, cut):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,tol", [("f64", 1e-13), ("f32", 1e-5)])
-def test_pml40_factor_backward_error(device, precision, tol):
+@pytest.mark.parametrize("precision,front_type,tol", [("f64", "ldl", 1e-13), ("f32", "ldl", 1e-5),
+                                                      ("f64", "block_ldl", 1e-10)])
+def test_pml40_factor_backward_error(device, precision, front_type, tol):
     # fronts up to 2,360 x 760, whose TRSMs (C = A Linv^T in place) span many
     # row tiles: the in-place product must keep a row block's columns in one
-    # tile (two 32-wide tile columns raced here, backward error 1e-5 in FP64)
+    # tile (two 32-wide tile columns raced here, backward error 1e-5 in FP64);
+    # the solves' update-row products (K = u up to 1,600) are split in K
     ip, ind, v, dims = problem("pml", 40, pml=5)
     N = ip.size - 1
-    sym, ft = gpu_tree(ip, ind, v, dims, 128, is_complex=True, precision=precision)
+    sym, ft = gpu_tree(ip, ind, v, dims, 128, is_complex=True, precision=precision,
+                       front_type=front_type)
     rng = np.random.default_rng(5)
     b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
     x = to_dev(b)
