@@ -515,7 +515,7 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
 }
 
 // ---------------------------------------------------------------------------
-// Clearing of update matrices (one 1024-element chunk per workgroup)
+// Clearing of update matrices (one ZERO_CHUNK-entry chunk per workgroup)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_zero_segments(const ZeroSeg<T>* __restrict__ segs,
@@ -524,11 +524,32 @@ __global__ __launch_bounds__(256) void k_zero_segments(const ZeroSeg<T>* __restr
     const int g = blockIdx.x;
     const int si = find_segment(chunk_start, nseg, g);
     const ZeroSeg<T> sg = segs[si];
-    const int64_t base = (int64_t)(g - chunk_start[si]) * 1024;
-    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
-        const int64_t e = base + i;
-        if (e < sg.n) sg.p[e] = zero<T>();
-    }
+    const int64_t base = (int64_t)(g - chunk_start[si]) * ZERO_CHUNK;
+    const int64_t end = min((int64_t)ZERO_CHUNK, sg.n - base);
+    T* p = sg.p + base;
+    for (int64_t i = threadIdx.x; i < end; i += blockDim.x) gstore(p + i, zero<T>());
+}
+
+// one wave per column, four per workgroup
+template <class T>
+__global__ __launch_bounds__(256) void k_zero_lower(const ZeroLowerDesc<T>* __restrict__ descs,
+                                                    const int* __restrict__ col_start, int nd,
+                                                    int ncols) {
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= ncols) return;
+    const int c = find_segment(col_start, nd, g);
+    const ZeroLowerDesc<T> d = descs[c];
+    const int j = g - col_start[c];
+    T* col = d.p + (int64_t)j * d.ld;
+    for (int i = j + (threadIdx.x & 63); i < d.rows; i += 64) gstore(col + i, zero<T>());
+}
+
+template <class T>
+void launch_zero_lower(const ZeroLowerDesc<T>* d_descs, const int* d_col_start, int ndesc,
+                       int ncols, hipStream_t s) {
+    if (ncols <= 0 || ndesc <= 0) return;
+    hipLaunchKernelGGL(k_zero_lower<T>, dim3((ncols + 3) / 4), dim3(256), 0, s, d_descs,
+                       d_col_start, ndesc, ncols);
 }
 
 template <class T>
@@ -617,13 +638,18 @@ void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bo
 }
 
 // ---------------------------------------------------------------------------
-// Extend-add: one workgroup per (child, update column). Children of one
-// slot have distinct parents, so no two workgroups touch the same entry.
+// Extend-add: one wave per (child, update column), four per workgroup (most
+// update columns are short: a workgroup each would make the launch
+// dispatch-bound). Children of one slot have distinct parents, so no two
+// waves touch the same entry.
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restrict__ descs,
-                                                    const int* __restrict__ col_start, int nd) {
-    const int g = blockIdx.x;
+                                                    const int* __restrict__ col_start, int nd,
+                                                    int ncols) {
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= ncols) return;
+    const int lane = threadIdx.x & 63;
     const int c = find_segment(col_start, nd, g);
     const ExtendDesc<T> d = descs[c];
     const int j = g - col_start[c];
@@ -632,7 +658,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restr
     T* dst;
     if (J < d.np) dst = d.Lp + (int64_t)J * d.mp;
     else dst = d.Up + (int64_t)(J - d.np) * d.up - d.np;
-    for (int i = j + threadIdx.x; i < d.uc; i += blockDim.x) {
+    for (int i = j + lane; i < d.uc; i += 64) {
         T* p = dst + d.rel[i];
         *p = *p + col[i];
     }
@@ -642,8 +668,8 @@ template <class T>
 void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int ndesc,
                        int ncols, hipStream_t s) {
     if (ncols <= 0) return;
-    hipLaunchKernelGGL(k_extend_add<T>, dim3(ncols), dim3(256), 0, s, d_descs, d_col_start,
-                       ndesc);
+    hipLaunchKernelGGL(k_extend_add<T>, dim3((ncols + 3) / 4), dim3(256), 0, s, d_descs,
+                       d_col_start, ndesc, ncols);
 }
 
 // ---------------------------------------------------------------------------
@@ -1067,6 +1093,7 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
     template void launch_zero_segments<T>(const ZeroSeg<T>*, const int*, int, int, hipStream_t); \
     template void launch_copy2d<T>(const CopyDesc<T>*, const int*, int, int, hipStream_t);       \
     template void launch_reduce_slices<T>(const ReduceDesc<T>*, const int*, int, int, hipStream_t); \
+    template void launch_zero_lower<T>(const ZeroLowerDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \

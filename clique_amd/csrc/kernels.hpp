@@ -118,6 +118,18 @@ struct ZeroSeg {
     T* p;
     int64_t n;
 };
+// Lower trapezoid of a column-major matrix: rows j..rows-1 of column j
+template <class T>
+struct ZeroLowerDesc {
+    T* p;
+    int64_t ld;
+    int rows, cols;
+};
+
+// Entries one workgroup clears (a launch has one workgroup per chunk of each
+// segment): large enough that the segment lookup and workgroup dispatch are
+// small next to the stores.
+constexpr int ZERO_CHUNK = 16384;
 
 // ---- launchers (stream-ordered, asynchronous) ----------------------------------
 // Clears every segment; chunk_start[i] = number of 1024-element chunks before
@@ -135,6 +147,9 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
 template <class T>
 void launch_copy2d(const CopyDesc<T>* d_descs, const int* d_chunk_start, int ndesc, int nchunks,
                    hipStream_t s);
+template <class T>
+void launch_zero_lower(const ZeroLowerDesc<T>* d_descs, const int* d_col_start, int ndesc,
+                       int ncols, hipStream_t s);
 template <class T>
 void launch_reduce_slices(const ReduceDesc<T>* d_descs, const int* d_chunk_start, int ndesc,
                           int nchunks, hipStream_t s);

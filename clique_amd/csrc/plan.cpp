@@ -110,7 +110,7 @@ struct IntervalAlloc {
 };
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
-                S_RECORD, S_WAIT, S_REDUCE };
+                S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER };
 
 struct Step {
     int kind;
@@ -137,6 +137,7 @@ const char* step_name(int kind) {
         case 6: return "zero_update";
         case 8: return "copy";
         case 11: return "solve_reduce";
+        case 12: return "zero_update";
         default: return "other";
     }
 }
@@ -169,7 +170,7 @@ struct StepBuilder {
             if (z.n <= 0) continue;
             keep.push_back(z);
             start.push_back(chunks);
-            chunks += (int)((z.n + 1023) / 1024);
+            chunks += (int)((z.n + ZERO_CHUNK - 1) / ZERO_CHUNK);
         }
         if (keep.empty()) return;
         steps.push_back(Step{S_ZERO, arena.add(keep), arena.add(start), (int)keep.size(), chunks,
@@ -440,20 +441,23 @@ struct Numeric : NumericBase {
         for (int h = 0; h < P.numLevels; ++h) {
             const auto& lv = P.levels[h];
             {
-                std::vector<ZeroSeg<T>> zs;
+                // only the lower triangle of an update matrix is ever written
+                // (Schur update, extend-add) or read (the parent's extend-add)
+                std::vector<ZeroLowerDesc<T>> zs;
                 std::vector<int> start;
-                int chunks = 0;
+                int cols = 0;
+                double by = 0.0;
                 for (int s : lv) {
                     const FrontLayout& f = P.fronts[s];
-                    const int64_t n = (int64_t)f.u * f.u;
-                    if (n == 0) continue;
-                    zs.push_back(ZeroSeg<T>{U(s), n});
-                    start.push_back(chunks);
-                    chunks += (int)((n + 1023) / 1024);
+                    if (f.u == 0) continue;
+                    zs.push_back(ZeroLowerDesc<T>{U(s), f.u, f.u, f.u});
+                    start.push_back(cols);
+                    cols += f.u;
+                    by += 0.5 * f.u * (f.u + 1.0) * sizeof(T);
                 }
                 if (!zs.empty())
-                    steps.push_back(Step{S_ZERO, arena.add(zs), arena.add(start), (int)zs.size(),
-                                          chunks, false});
+                    steps.push_back(Step{S_ZLOWER, arena.add(zs), arena.add(start), (int)zs.size(),
+                                          cols, false, 0.0, by});
             }
             for (int slot = 0; slot < 2; ++slot) {
                 std::vector<ExtendDesc<T>> ds;
@@ -816,7 +820,7 @@ struct Numeric : NumericBase {
                         if (n == 0) continue;
                         zs.push_back(ZeroSeg<T>{Wp(s), n});
                         zst.push_back(chunks);
-                        chunks += (int)((n + 1023) / 1024);
+                        chunks += (int)((n + ZERO_CHUNK - 1) / ZERO_CHUNK);
                     }
                     if (!zs.empty())
                         steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
@@ -970,6 +974,10 @@ struct Numeric : NumericBase {
                 case S_ZERO:
                     launch_zero_segments<T>(a.at<ZeroSeg<T>>(s.desc), a.at<int>(s.start), s.ndesc,
                                             s.nwork, st);
+                    break;
+                case S_ZLOWER:
+                    launch_zero_lower<T>(a.at<ZeroLowerDesc<T>>(s.desc), a.at<int>(s.start),
+                                         s.ndesc, s.nwork, st);
                     break;
                 case S_EXTEND:
                     launch_extend_add<T>(a.at<ExtendDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
