@@ -814,46 +814,59 @@ void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_st
 }
 
 // ---------------------------------------------------------------------------
-// Solve helpers
+// Solve helpers. Column j of an N x k block is blockIdx.y (k <= 65535 is
+// checked by the host), rows are strided over blockIdx.x: no 64-bit index
+// division per element.
 // ---------------------------------------------------------------------------
-template <class T, class V>
-__global__ void k_pull(const V* __restrict__ x, int64_t ldx, const int* __restrict__ inv,
-                       T* __restrict__ xn, int N, int k) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int p = (int)(g % N), j = (int)(g / N);
-        xn[g] = cvt<T>(x[inv[p] + j * ldx]);
-    }
-}
-template <class T, class V>
-__global__ void k_push(const T* __restrict__ xn, const int* __restrict__ inv, V* __restrict__ x,
-                       int64_t ldx, int N, int k) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int p = (int)(g % N), j = (int)(g / N);
-        x[inv[p] + j * ldx] = cvt<V>(xn[g]);
-    }
-}
-
 static inline unsigned grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
     if (b < 1) b = 1;
     return (unsigned)b;
 }
+static inline dim3 grid_cols(int N, int k) {
+    unsigned gx = grid_for(N);
+    if (gx > 512) gx = 512;
+    return dim3(gx, (unsigned)k);
+}
+
+// xn(map[i], j) = x(i, j): reads of x (the wide type) coalesced, map[i] the
+// nodal index of original row i
+template <class T, class V>
+__global__ void k_pull(const V* __restrict__ x, int64_t ldx, const int* __restrict__ map,
+                       T* __restrict__ xn, int N) {
+    const int j = blockIdx.y;
+    const V* xc = x + (int64_t)j * ldx;
+    T* yc = xn + (int64_t)j * N;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        yc[map[i]] = cvt<T>(xc[i]);
+}
+// x(i, j) = xn(map[i], j): writes of x coalesced
+template <class T, class V>
+__global__ void k_push(const T* __restrict__ xn, const int* __restrict__ map, V* __restrict__ x,
+                       int64_t ldx, int N) {
+    const int j = blockIdx.y;
+    const T* xc = xn + (int64_t)j * N;
+    V* yc = x + (int64_t)j * ldx;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        yc[i] = cvt<V>(xc[map[i]]);
+}
 
 template <class T, class V>
-void launch_pull(const V* x, int64_t ldx, const int* inv_map, T* xn, int N, int k, hipStream_t s) {
-    hipLaunchKernelGGL((k_pull<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, x, ldx,
-                       inv_map, xn, N, k);
+void launch_pull(const V* x, int64_t ldx, const int* map, T* xn, int N, int k, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL((k_pull<T, V>), grid_cols(N, k), dim3(256), 0, s, x, ldx, map, xn, N);
 }
 template <class T, class V>
-void launch_push(const T* xn, const int* inv_map, V* x, int64_t ldx, int N, int k, hipStream_t s) {
-    hipLaunchKernelGGL((k_push<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, xn,
-                       inv_map, x, ldx, N, k);
+void launch_push(const T* xn, const int* map, V* x, int64_t ldx, int N, int k, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL((k_push<T, V>), grid_cols(N, k), dim3(256), 0, s, xn, map, x, ldx, N);
 }
+
+// Right-hand-side columns a gather / solve extend-add thread handles: all of
+// a thread's loads are issued before its stores (the destination may alias
+// the source pool, so the compiler could not reorder them itself).
+constexpr int RHS_CHUNK = 8;
 
 template <class T>
 __global__ void k_gather(const SolveSeg* __restrict__ segs, const int* __restrict__ row_start,
@@ -861,19 +874,26 @@ __global__ void k_gather(const SolveSeg* __restrict__ segs, const int* __restric
                          int N, int k, const int* __restrict__ lowpool) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nrows) return;
+    const int j0 = blockIdx.y * RHS_CHUNK;
     const int si = find_segment(row_start, nseg, g);
     const SolveSeg sg = segs[si];
     const int i = g - row_start[si];
-    T* w = wpool + sg.w * k;
-    const int src = lowpool[sg.low + i];
-    for (int j = 0; j < k; ++j) w[i + (int64_t)j * sg.u] = xn[src + (int64_t)j * N];
+    T* w = wpool + sg.w * k + i + (int64_t)j0 * sg.u;
+    const T* src = xn + lowpool[sg.low + i] + (int64_t)j0 * N;
+    T v[RHS_CHUNK];
+#pragma unroll
+    for (int q = 0; q < RHS_CHUNK; ++q)
+        if (j0 + q < k) v[q] = src[(int64_t)q * N];
+#pragma unroll
+    for (int q = 0; q < RHS_CHUNK; ++q)
+        if (j0 + q < k) w[(int64_t)q * sg.u] = v[q];
 }
 template <class T>
 void launch_gather(const SolveSeg* segs, const int* row_start, int nseg, int nrows, T* wpool,
                    const T* xn, int N, int k, const int* lowpool, hipStream_t s) {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_gather<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, segs, row_start,
-                       nseg, nrows, wpool, xn, N, k, lowpool);
+    if (nrows <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_gather<T>, dim3((nrows + 255) / 256, (k + RHS_CHUNK - 1) / RHS_CHUNK),
+                       dim3(256), 0, s, segs, row_start, nseg, nrows, wpool, xn, N, k, lowpool);
 }
 
 template <class T>
@@ -882,10 +902,11 @@ __global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __
                                const int* __restrict__ relpool, int k) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nrows) return;
+    const int j0 = blockIdx.y * RHS_CHUNK;
     const int c = find_segment(row_start, nd, g);
     const SolveExtend d = ds[c];
     const int i = g - row_start[c];
-    const T* wc = wpool + d.wc * k;
+    const T* wc = wpool + d.wc * k + i + (int64_t)j0 * d.uc;
     const int I = relpool[d.rel + i];
     T* dst;
     int64_t ld;
@@ -896,80 +917,181 @@ __global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __
         dst = wpool + d.wp * k + (I - d.np);
         ld = d.up;
     }
-    for (int j = 0; j < k; ++j) {
-        T* p = dst + (int64_t)j * ld;
-        *p = *p + wc[i + (int64_t)j * d.uc];
-    }
+    dst += (int64_t)j0 * ld;
+    T a[RHS_CHUNK], b[RHS_CHUNK];
+#pragma unroll
+    for (int q = 0; q < RHS_CHUNK; ++q)
+        if (j0 + q < k) {
+            a[q] = wc[(int64_t)q * d.uc];
+            b[q] = dst[(int64_t)q * ld];
+        }
+#pragma unroll
+    for (int q = 0; q < RHS_CHUNK; ++q)
+        if (j0 + q < k) dst[(int64_t)q * ld] = b[q] + a[q];
 }
 template <class T>
 void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
                          T* wpool, T* xn, int N, const int* relpool, int k, hipStream_t s) {
-    if (nrows <= 0) return;
-    hipLaunchKernelGGL(k_solve_extend<T>, dim3((nrows + 255) / 256), dim3(256), 0, s, d,
-                       row_start, nd, nrows, wpool, xn, N, relpool, k);
+    if (nrows <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_solve_extend<T>,
+                       dim3((nrows + 255) / 256, (k + RHS_CHUNK - 1) / RHS_CHUNK), dim3(256), 0, s,
+                       d, row_start, nd, nrows, wpool, xn, N, relpool, k);
 }
 
+// One thread per row: the pivot's reciprocal once, then its k columns
 template <class T, class V>
 __global__ void k_diag_solve(T* __restrict__ xn, const V* __restrict__ D, int N, int k,
                              int* status) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int p = (int)(g % N);
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         const V dv = D[p];
         if (is_zero_or_nonfinite(dv)) record_status(status, 2, -1, p);
-        xn[g] = cvt<T>(cvt<V>(xn[g]) / dv);
+        const V r = one<V>() / dv;
+        T* xc = xn + p;
+        for (int j = 0; j < k; ++j) xc[(int64_t)j * N] = cvt<T>(cvt<V>(xc[(int64_t)j * N]) * r);
     }
 }
 template <class T, class V>
 void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream_t s) {
-    hipLaunchKernelGGL((k_diag_solve<T, V>), dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, xn,
-                       D, N, k, d_status);
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL((k_diag_solve<T, V>), dim3(grid_for(N)), dim3(256), 0, s, xn, D, N, k,
+                       d_status);
 }
 
 // ---------------------------------------------------------------------------
 // FP64 sparse kernels
 // ---------------------------------------------------------------------------
-// y := alpha A x + beta y over CSR rows; one thread per (row, 8 right-hand
-// sides): the row's indices and values are read once per chunk of columns.
-constexpr int SPMV_COLS = 8;
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// y := alpha A x + beta b over CSR rows (b = y when null); with colmax,
+// colmax[j] = max(colmax[j], max_i |y(i, j)|). One thread per (row, 8
+// right-hand sides): the row's indices and values are read once per chunk
+// of columns, and for rows of at most 8 entries all of the thread's loads
+// are issued before its products.
+constexpr int SPMV_COLS = 8, SPMV_ROWNZ = 8;
 template <class V>
-__global__ void k_spmv(int N, const int64_t* __restrict__ rowoff, const int* __restrict__ cols,
-                       const V* __restrict__ vals, double alpha, const V* __restrict__ x,
-                       int64_t ldx, double beta, V* __restrict__ y, int64_t ldy, int k) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int j0 = blockIdx.y * SPMV_COLS;
+__global__ __launch_bounds__(256) void k_spmv(int N, const int64_t* __restrict__ rowoff,
+                                              const int* __restrict__ cols,
+                                              const V* __restrict__ vals, double alpha,
+                                              const V* __restrict__ x, int64_t ldx, double beta,
+                                              const V* __restrict__ b, int64_t ldb,
+                                              V* __restrict__ y, int64_t ldy, int k,
+                                              unsigned long long* colmax) {
+    __shared__ double red[SPMV_COLS][4];
+    // one-dimensional grid of (row block, column chunk) pairs, remapped so
+    // each XCD runs a contiguous range of row blocks of one column chunk:
+    // the stencil's neighbours in x then come from that XCD's L2
+    const int rb = (N + 255) / 256;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = (t % rb) * 256 + threadIdx.x;
+    const int j0 = (t / rb) * SPMV_COLS;
     const int nj = min(SPMV_COLS, k - j0);
-    V acc[SPMV_COLS];
+    double m[SPMV_COLS];
 #pragma unroll
-    for (int q = 0; q < SPMV_COLS; ++q) acc[q] = V{};
-    for (int64_t e = rowoff[i]; e < rowoff[i + 1]; ++e) {
-        const V a = vals[e];
-        const V* xc = x + cols[e] + j0 * ldx;
+    for (int q = 0; q < SPMV_COLS; ++q) m[q] = 0.0;
+    if (i < N) {
+        V acc[SPMV_COLS];
 #pragma unroll
-        for (int q = 0; q < SPMV_COLS; ++q)
-            if (q < nj) acc[q] = acc[q] + a * xc[q * ldx];
+        for (int q = 0; q < SPMV_COLS; ++q) acc[q] = V{};
+        const int64_t e0 = rowoff[i], e1 = rowoff[i + 1];
+        const int ne = (int)(e1 - e0);
+        if (ne <= SPMV_ROWNZ) {
+            int c[SPMV_ROWNZ];
+            V a[SPMV_ROWNZ];
+#pragma unroll
+            for (int u = 0; u < SPMV_ROWNZ; ++u)
+                if (u < ne) {
+                    c[u] = cols[e0 + u];
+                    a[u] = vals[e0 + u];
+                }
+#pragma unroll
+            for (int q = 0; q < SPMV_COLS; ++q)
+                if (q < nj) {
+                    const V* xc = x + (int64_t)(j0 + q) * ldx;
+                    V sum = V{};
+#pragma unroll
+                    for (int u = 0; u < SPMV_ROWNZ; ++u)
+                        if (u < ne) sum = sum + a[u] * xc[c[u]];
+                    acc[q] = sum;
+                }
+        } else {
+            for (int64_t e = e0; e < e1; ++e) {
+                const V a = vals[e];
+                const V* xc = x + cols[e] + (int64_t)j0 * ldx;
+#pragma unroll
+                for (int q = 0; q < SPMV_COLS; ++q)
+                    if (q < nj) acc[q] = acc[q] + a * xc[q * ldx];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < SPMV_COLS; ++q) {
+            if (q >= nj) break;
+            V* py = y + i + (int64_t)(j0 + q) * ldy;
+            V out = scale(acc[q], alpha);
+            if (beta != 0.0) out = out + scale(b ? b[i + (int64_t)(j0 + q) * ldb] : *py, beta);
+            *py = out;
+            m[q] = abs1(out);
+        }
     }
+    if (colmax) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < SPMV_COLS; ++q) {
-        if (q >= nj) break;
-        V* py = y + i + (j0 + q) * ldy;
-        V out = scale(acc[q], alpha);
-        if (beta != 0.0) out = out + scale(*py, beta);
-        *py = out;
+        for (int q = 0; q < SPMV_COLS; ++q) {
+            double v = m[q];
+            for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+            if (lane == 0) red[q][wave] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < nj) {
+            const int q = threadIdx.x;
+            const double v = fmax(fmax(red[q][0], red[q][1]), fmax(red[q][2], red[q][3]));
+            atomic_max_nonneg(colmax + j0 + q, v);
+        }
     }
+}
+template <class V>
+void launch_spmv_ex(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
+                    const V* x, int64_t ldx, double beta, const V* b, int64_t ldb, V* y,
+                    int64_t ldy, int k, unsigned long long* colmax, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    const int64_t blocks = (int64_t)((N + 255) / 256) * ((k + SPMV_COLS - 1) / SPMV_COLS);
+    if (blocks > 0x7fffffff) throw Error(1, "SpMV grid too large");
+    hipLaunchKernelGGL(k_spmv<V>, dim3((unsigned)blocks), dim3(256), 0, s, N, rowoff, cols, vals,
+                       alpha, x, ldx, beta, b, ldb, y, ldy, k, colmax);
 }
 template <class V>
 void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
                  const V* x, int64_t ldx, double beta, V* y, int64_t ldy, int k, hipStream_t s) {
-    if (N <= 0 || k <= 0) return;
-    hipLaunchKernelGGL(k_spmv<V>, dim3((N + 255) / 256, (k + SPMV_COLS - 1) / SPMV_COLS), dim3(256),
-                       0, s, N, rowoff, cols, vals, alpha, x, ldx, beta, y, ldy, k);
+    launch_spmv_ex<V>(N, rowoff, cols, vals, alpha, x, ldx, beta, nullptr, 0, y, ldy, k, nullptr,
+                      s);
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
-    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+// y += x column by column, with colmax[j] = max(colmax[j], max_i |y(i, j)|)
+template <class V>
+__global__ __launch_bounds__(256) void k_axpy_abs(V* __restrict__ y, int64_t ldy,
+                                                  const V* __restrict__ x, int64_t ldx, int N,
+                                                  unsigned long long* colmax) {
+    __shared__ double red[4];
+    V* yc = y + (int64_t)blockIdx.y * ldy;
+    const V* xc = x + (int64_t)blockIdx.y * ldx;
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const V v = yc[i] + xc[i];
+        yc[i] = v;
+        m = fmax(m, abs1(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomic_max_nonneg(colmax + blockIdx.y, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+}
+template <class V>
+void launch_axpy_abs(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k,
+                     unsigned long long* colmax, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_axpy_abs<V>, grid_cols(N, k), dim3(256), 0, s, y, ldy, x, ldx, N, colmax);
 }
 
 template <class V>
@@ -1023,34 +1145,30 @@ void launch_col_abs_max(const V* x, int64_t ldx, int N, int k, unsigned long lon
 
 template <class V>
 __global__ void k_axpy_cols(V* __restrict__ y, int64_t ldy, const V* __restrict__ x, int64_t ldx,
-                            int N, int k) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(g % N), j = (int)(g / N);
-        y[i + j * ldy] = y[i + j * ldy] + x[i + j * ldx];
-    }
+                            int N) {
+    V* yc = y + (int64_t)blockIdx.y * ldy;
+    const V* xc = x + (int64_t)blockIdx.y * ldx;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        yc[i] = yc[i] + xc[i];
 }
 template <class V>
 void launch_axpy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s) {
-    hipLaunchKernelGGL(k_axpy_cols<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, y, ldy, x,
-                       ldx, N, k);
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_axpy_cols<V>, grid_cols(N, k), dim3(256), 0, s, y, ldy, x, ldx, N);
 }
 
 template <class V>
 __global__ void k_copy_cols(V* __restrict__ y, int64_t ldy, const V* __restrict__ x, int64_t ldx,
-                            int N, int k) {
-    const int64_t total = (int64_t)N * k;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(g % N), j = (int)(g / N);
-        y[i + j * ldy] = x[i + j * ldx];
-    }
+                            int N) {
+    V* yc = y + (int64_t)blockIdx.y * ldy;
+    const V* xc = x + (int64_t)blockIdx.y * ldx;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        yc[i] = xc[i];
 }
 template <class V>
 void launch_copy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s) {
-    hipLaunchKernelGGL(k_copy_cols<V>, dim3(grid_for((int64_t)N * k)), dim3(256), 0, s, y, ldy, x,
-                       ldx, N, k);
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_copy_cols<V>, grid_cols(N, k), dim3(256), 0, s, y, ldy, x, ldx, N);
 }
 
 template <class V>
@@ -1122,6 +1240,11 @@ CQ_INST_TV(cdouble, cdouble)
 #define CQ_INST_V(V)                                                                            \
     template void launch_spmv<V>(int, const int64_t*, const int*, const V*, double, const V*,   \
                                  int64_t, double, V*, int64_t, int, hipStream_t);              \
+    template void launch_spmv_ex<V>(int, const int64_t*, const int*, const V*, double,          \
+                                    const V*, int64_t, double, const V*, int64_t, V*, int64_t,  \
+                                    int, unsigned long long*, hipStream_t);                    \
+    template void launch_axpy_abs<V>(V*, int64_t, const V*, int64_t, int, int,                   \
+                                     unsigned long long*, hipStream_t);                        \
     template void launch_row_abs_sum_max<V>(int, const int64_t*, const V*, unsigned long long*, \
                                             hipStream_t);                                      \
     template void launch_col_abs_max<V>(const V*, int64_t, int, int, unsigned long long*,       \

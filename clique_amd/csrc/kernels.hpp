@@ -187,6 +187,16 @@ void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream
 template <class V>
 void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
                  const V* x, int64_t ldx, double beta, V* y, int64_t ldy, int k, hipStream_t s);
+// y := alpha A x + beta b (b == nullptr: beta y); colmax (optional):
+// colmax[j] = max(colmax[j], max_i |y(i, j)|) as FP64 bit patterns
+template <class V>
+void launch_spmv_ex(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
+                    const V* x, int64_t ldx, double beta, const V* b, int64_t ldb, V* y,
+                    int64_t ldy, int k, unsigned long long* colmax, hipStream_t s);
+// y += x with colmax[j] = max(colmax[j], max_i |y(i, j)|)
+template <class V>
+void launch_axpy_abs(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k,
+                     unsigned long long* colmax, hipStream_t s);
 template <class V>
 void launch_row_abs_sum_max(int N, const int64_t* rowoff, const V* vals, unsigned long long* out,
                             hipStream_t s);

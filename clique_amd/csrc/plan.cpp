@@ -1089,7 +1089,8 @@ struct Numeric : NumericBase {
     void solve(V* x, int64_t ldx, int k, bool nodal) {
         SolveSched& S = schedule(k);
         hipStream_t st = P.stream;
-        const int* map = P.dInvMap.as<int>();
+        if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
+        const int* map = P.dMap.as<int>();
         if (nodal) {
             if (!identity.p) {
                 std::vector<int> id(P.N);
@@ -1266,7 +1267,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     upload(dAsmDst, dst);
     upload(dRel, relAll);
     upload(dLow, lowAll);
-    upload(dInvMap, S.inverseMap);
+    upload(dMap, S.map);
     dStatus.alloc(4 * sizeof(int));
     dNorms.alloc(4096 * sizeof(unsigned long long));
     dVals.alloc(std::max<int64_t>(nnz, 1) * (complex ? 16 : 8));
@@ -1299,7 +1300,7 @@ void Plan::after_caller() {
 
 size_t Plan::device_bytes() const {
     size_t b = dRowOff.bytes + dCols.bytes + dAsmDst.bytes + dRel.bytes + dLow.bytes +
-               dInvMap.bytes + dStatus.bytes + dNorms.bytes + dVals.bytes;
+               dMap.bytes + dStatus.bytes + dNorms.bytes + dVals.bytes;
     if (num) b += num->device_bytes();
     return b;
 }
@@ -1447,17 +1448,15 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
         launch_col_abs_max<V>(b, ldb, N, k, nrm + 1, stream);
         launch_copy_cols<V>(x, ldx, b, ldb, N, k, stream);
         nm->solve(x, ldx, k, false);
+        launch_col_abs_max<V>(x, ldx, N, k, nrm + 1 + k, stream);
         std::vector<unsigned long long> h(3 * k + 1);
         int it = 0;
         bool conv = false;
         double eta = 0.0;
         for (;;) {
-            launch_copy_cols<V>(r, N, b, ldb, N, k, stream);
-            launch_spmv<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<V>(), -1.0, x, ldx,
-                           1.0, r, N, k, stream);
-            CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
-            launch_col_abs_max<V>(x, ldx, N, k, nrm + 1 + k, stream);
-            launch_col_abs_max<V>(r, N, N, k, nrm + 1 + 2 * k, stream);
+            // r = b - A x with |r| per column in one pass; |x| came with x
+            launch_spmv_ex<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<V>(), -1.0, x,
+                              ldx, 1.0, b, ldb, r, N, k, nrm + 1 + 2 * k, stream);
             CQ_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(unsigned long long) * (3 * k + 1),
                                   hipMemcpyDeviceToHost, stream));
             CQ_HIP(hipStreamSynchronize(stream));
@@ -1474,7 +1473,8 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
             if (eta <= tol) { conv = true; break; }
             if (it >= maxIters) break;
             nm->solve(r, N, k, false);
-            launch_axpy_cols<V>(x, ldx, r, N, N, k, stream);
+            CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
+            launch_axpy_abs<V>(x, ldx, r, N, N, k, nrm + 1 + k, stream);
             ++it;
         }
         *itersOut = it;

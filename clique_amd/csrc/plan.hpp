@@ -90,7 +90,7 @@ public:
     std::vector<hipEvent_t> depEvents;
     void ensure_dep_events(int n);
 
-    DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dInvMap, dVals, dStatus, dNorms;
+    DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dMap, dVals, dStatus, dNorms;
     int precision = -1, frontType = 0;
     bool factored = false;
     double normA = 0.0;
