@@ -31,6 +31,13 @@ template <> __device__ __forceinline__ double make_t<double>(double r, double) {
 template <> __device__ __forceinline__ cfloat make_t<cfloat>(float r, float i) { return {r, i}; }
 template <> __device__ __forceinline__ cdouble make_t<cdouble>(double r, double i) { return {r, i}; }
 
+// 1 / a with one real division
+__device__ __forceinline__ double pivot_inverse(double a) { return 1.0 / a; }
+__device__ __forceinline__ cdouble pivot_inverse(cdouble a) {
+    const double s = 1.0 / (a.x * a.x + a.y * a.y);
+    return {a.x * s, -a.y * s};
+}
+
 __device__ __forceinline__ double real_part_only(double a) { return a; }
 __device__ __forceinline__ cdouble real_part_only(cdouble a) { return {a.x, 0.0}; }
 
@@ -436,6 +443,17 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
     const W* scaleN = reinterpret_cast<const W*>(d.scaleN);
     const bool lower = d.flags & G_LOWER, accum = d.flags & G_ACCUM, neg = d.flags & G_NEG;
     const bool symm = lower && (d.flags & G_SYMM);
+    // a lane's output columns do not depend on r: their pivots and the
+    // pivots' reciprocals are formed once per 16-column block
+    W sN[BN], rN[BN];
+    if (scaleN) {
+#pragma unroll
+        for (int b = 0; b < BN; ++b) {
+            const int gj = col0 + wj * WTN + b * 16 + (lane & 15);
+            sN[b] = gload(scaleN + (gj < d.N ? gj : 0));
+            rN[b] = pivot_inverse(sN[b]);
+        }
+    }
 #pragma unroll
     for (int a = 0; a < BM; ++a)
 #pragma unroll
@@ -453,11 +471,11 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
                 else
                     v = make_t<T>(accR[a][b][r], CPX ? accI[a][b][r] : R(0));
                 if (scaleN) {
-                    v = cvt<T>(cvt<W>(v) / scaleN[gj]);
+                    v = cvt<T>(cvt<W>(v) * rN[b]);
                     // W = L D from the rounded L, exactly as a pivot-scaled
                     // update would form it, so updates from W stay consistent
                     // with the stored factor
-                    if (d.C2) gstore(d.C2 + gi + (int64_t)gj * d.ldc, v * cvt<T>(scaleN[gj]));
+                    if (d.C2) gstore(d.C2 + gi + (int64_t)gj * d.ldc, v * cvt<T>(sN[b]));
                 }
                 if (neg) v = -v;
                 T* p = d.C + gi + (int64_t)gj * d.ldc;
@@ -689,11 +707,6 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 // lower elements and r < q only upper ones; the slot r == q is lower in the
 // threads with ti >= tj.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double pivot_inverse(double a) { return 1.0 / a; }
-__device__ __forceinline__ cdouble pivot_inverse(cdouble a) {
-    const double s = 1.0 / (a.x * a.x + a.y * a.y);
-    return {a.x * s, -a.y * s};
-}
 
 // (Two waves per SIMD are plenty for this latency-bound kernel; the complex
 // instantiations need the registers.)
