@@ -371,8 +371,9 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
         constexpr int buf = decltype(bufc)::value;
         if (c + 1 < nk) load(kbeg + (c + 1) * TK);
         // raised wave priority while feeding the MFMAs (measured: +3 % for
-        // complex FP64, neutral elsewhere)
-        __builtin_amdgcn_s_setprio(1);
+        // complex FP64, neutral elsewhere); the TRSMs, on the factorization's
+        // critical path beside the lookahead updates, one level higher
+        __builtin_amdgcn_s_setprio(ROLE == 0 ? 2 : 1);
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
@@ -431,7 +432,7 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
                     }
             }
         }
-        __builtin_amdgcn_s_setprio(0);
+        __builtin_amdgcn_s_setprio(ROLE == 0 ? 1 : 0);
         if (c + 1 < nk) store(std::integral_constant<int, buf ^ 1>{});
         __syncthreads();
     };
@@ -718,6 +719,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) voi
     __shared__ W dinv[NB];
     __shared__ W dval[NB];
 
+    // the diagonal blocks are the factorization's critical path while the
+    // lookahead's updates run beside them on the same CUs
+    __builtin_amdgcn_s_setprio(3);
     const DiagDesc<T> d = descs[blockIdx.x];
     const int b = d.b, tid = threadIdx.x;
     const int ti = tid & 15, tj = tid >> 4;
