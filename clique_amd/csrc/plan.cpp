@@ -317,7 +317,15 @@ struct StepBuilder {
 // Numeric state for one working type T (float, double, cfloat, cdouble)
 // ---------------------------------------------------------------------------
 // Outer panel width of the front factorization (a multiple of NB).
-constexpr int OB = 256;
+constexpr int OB_DEFAULT = 256;
+static int outer_panel() {
+    static const int ob = [] {
+        const char* e = std::getenv("CLIQG_OUTER_PANEL");
+        const int v = e ? std::atoi(e) : OB_DEFAULT;
+        return (v >= NB && v % NB == 0) ? v : OB_DEFAULT;
+    }();
+    return ob;
+}
 
 template <class T>
 struct Numeric : NumericBase {
@@ -504,6 +512,7 @@ struct Numeric : NumericBase {
             // Two-level panels: 64-wide diagonal blocks whose updates stay
             // inside an OB-wide outer panel, then one rank-OB update of the
             // front's trailing columns (K = OB instead of K = 64 passes).
+            const int OB = outer_panel();
             for (int k0 = 0; k0 < maxn; k0 += OB) {
                 for (int k = k0; k < std::min(maxn, k0 + OB); k += NB) {
                     std::vector<DiagDesc<T>> dd;
