@@ -322,7 +322,7 @@ def main():
     g = prof32["gemm_update"]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
     traffic, traffic_src = measured_traffic(args.config, args.n)
-    solve_names = ("solve_gemm", "solve_extend_add", "solve_gather", "solve_zero")
+    solve_names = ("solve_gemm", "solve_extend_add", "solve_gather", "solve_zero", "solve_reduce")
 
     def families(prof, solve):
         # one step (factor + refinement)

@@ -1025,8 +1025,11 @@ struct Numeric : NumericBase {
                 const int kind = (s.kind == S_GEMM && S) ? 7 : s.kind;
                 // factor GEMMs by instantiation: pivot-scaled updates (Schur,
                 // panel and outer-panel updates) apart from the TRSM-like rest
+                // (32x32-tile updates apart: a different kernel)
                 const char* name = (s.kind == S_GEMM && !S)
-                                       ? ((s.update || s.scaled) ? "gemm_update" : "gemm_trsm")
+                                       ? ((s.update || s.scaled)
+                                              ? (s.variant == 5 ? "gemm_update_small" : "gemm_update")
+                                              : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
                                                              : step_name(kind);
                 P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1, s.ndesc, s.nwork,
