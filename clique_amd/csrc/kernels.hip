@@ -671,7 +671,7 @@ __global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restr
     const int lane = threadIdx.x & 63;
     const int c = find_segment(col_start, nd, g);
     const ExtendDesc<T> d = descs[c];
-    const int j = g - col_start[c];
+    const int j = g - col_start[c] + d.j0;
     const int J = d.rel[j];
     const T* col = d.Uc + (int64_t)j * d.uc;
     T* dst;

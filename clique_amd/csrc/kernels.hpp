@@ -79,6 +79,7 @@ struct ExtendDesc {
     T* Lp;            // parent frontL (mp x np, ld mp)
     T* Up;            // parent update matrix (up x up, ld up)
     int uc, np, mp, up;
+    int j0;           // first child column of this launch (its count is in col_start)
 };
 
 // One front of a diagonal-block factorization step.

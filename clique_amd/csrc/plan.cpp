@@ -448,16 +448,52 @@ struct Numeric : NumericBase {
         auto Tp = [&](int s) { return btmp.as<T>() + tmpOff[s]; };
         for (int h = 0; h < P.numLevels; ++h) {
             const auto& lv = P.levels[h];
+            // Extend-add in two passes (ldl/local.hpp:60-102 adds each child's
+            // update matrix into the parent front in one): the child columns
+            // that map into the parent's own columns (L part, a prefix since
+            // the relative indices increase) before the panels, the rest
+            // (update part) after the Schur update, which then writes the
+            // update matrix instead of accumulating into a cleared one.
+            auto extend = [&](bool upart) {
+                for (int slot = 0; slot < 2; ++slot) {
+                    std::vector<ExtendDesc<T>> ds;
+                    std::vector<int> start;
+                    int total = 0;
+                    double by = 0.0;
+                    for (int p : lv) {
+                        const FrontLayout& fp = P.fronts[p];
+                        if (fp.nchild <= slot) continue;
+                        const int c = fp.child[slot];
+                        const FrontLayout& fc = P.fronts[c];
+                        if (fc.u == 0) continue;
+                        const int* r = P.relHost.data() + fc.rel;
+                        const int js = (int)(std::lower_bound(r, r + fc.u, fp.n) - r);
+                        const int j0 = upart ? js : 0, nc = upart ? fc.u - js : js;
+                        if (nc == 0) continue;
+                        ds.push_back(ExtendDesc<T>{U(c), P.dRel.as<int>() + fc.rel, L(p), U(p),
+                                                   fc.u, fp.n, fp.m, fp.u, j0});
+                        start.push_back(total);
+                        total += nc;
+                        // entries of columns j0..j0+nc-1 of a lower triangle of order uc
+                        const double e = (double)nc * fc.u - ((double)(j0 + nc) * (j0 + nc - 1) -
+                                                               (double)j0 * (j0 - 1)) / 2.0;
+                        by += 3.0 * e * sizeof(T);
+                    }
+                    if (!ds.empty())
+                        steps.push_back(Step{S_EXTEND, arena.add(ds), arena.add(start),
+                                              (int)ds.size(), total, false, 0.0, by});
+                }
+            };
             {
-                // only the lower triangle of an update matrix is ever written
-                // (Schur update, extend-add) or read (the parent's extend-add)
+                // update matrices of fronts without columns get no Schur
+                // update: clear them (only their lower triangle is read)
                 std::vector<ZeroLowerDesc<T>> zs;
                 std::vector<int> start;
                 int cols = 0;
                 double by = 0.0;
                 for (int s : lv) {
                     const FrontLayout& f = P.fronts[s];
-                    if (f.u == 0) continue;
+                    if (f.u == 0 || f.n > 0) continue;
                     zs.push_back(ZeroLowerDesc<T>{U(s), f.u, f.u, f.u});
                     start.push_back(cols);
                     cols += f.u;
@@ -467,28 +503,7 @@ struct Numeric : NumericBase {
                     steps.push_back(Step{S_ZLOWER, arena.add(zs), arena.add(start), (int)zs.size(),
                                           cols, false, 0.0, by});
             }
-            for (int slot = 0; slot < 2; ++slot) {
-                std::vector<ExtendDesc<T>> ds;
-                std::vector<int> start;
-                int total = 0;
-                for (int p : lv) {
-                    const FrontLayout& fp = P.fronts[p];
-                    if (fp.nchild <= slot) continue;
-                    const int c = fp.child[slot];
-                    const FrontLayout& fc = P.fronts[c];
-                    if (fc.u == 0) continue;
-                    ds.push_back(ExtendDesc<T>{U(c), P.dRel.as<int>() + fc.rel, L(p), U(p), fc.u,
-                                               fp.n, fp.m, fp.u});
-                    start.push_back(total);
-                    total += fc.u;
-                }
-                if (!ds.empty()) {
-                    double by = 0.0;
-                    for (const auto& x : ds) by += 3.0 * x.uc * (x.uc + 1.0) / 2.0 * sizeof(T);
-                    steps.push_back(Step{S_EXTEND, arena.add(ds), arena.add(start),
-                                          (int)ds.size(), total, false, 0.0, by});
-                }
-            }
+            extend(false);
             if (blockLdl) {
                 std::vector<CopyDesc<T>> cs;
                 for (int s : lv) {
@@ -603,10 +618,10 @@ struct Numeric : NumericBase {
                 if (f.u == 0 || f.n == 0) continue;
                 T* l21 = L(s) + f.n;
                 syrk.push_back(GemmDesc<T>{Wf(s) + f.n, l21, U(s), nullptr, nullptr, f.m, f.m, f.u,
-                                           f.u, f.u, f.n,
-                                           G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+                                           f.u, f.u, f.n, G_TRANSB | G_LOWER | G_NEG | herm});
             }
             sb.update(syrk);
+            extend(true);
             if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp);
             else wide_inverse(sb, lv);
         }
@@ -1278,6 +1293,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
     upload(dCols, colv);
     upload(dAsmDst, dst);
     upload(dRel, relAll);
+    relHost = std::move(relAll);
     upload(dLow, lowAll);
     upload(dMap, S.map);
     dStatus.alloc(4 * sizeof(int));

@@ -91,6 +91,7 @@ public:
     void ensure_dep_events(int n);
 
     DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dMap, dVals, dStatus, dNorms;
+    std::vector<int> relHost;   // relative indices of every front (host copy of dRel)
     int precision = -1, frontType = 0;
     bool factored = false;
     double normA = 0.0;

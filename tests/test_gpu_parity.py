@@ -121,6 +121,28 @@ def test_pml40_factor_backward_error(device, precision, front_type, tol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_helmholtz64_split_k_solves(device, precision):
+    # the 64^3 Helmholtz operator (N = 262,144): fronts up to 4,096 wide whose
+    # update-row products (K = n or u >= 1,024, few tiles) are split in K in
+    # both sweeps; FP64 solves to rounding, FP32 refines to 1e-10
+    ip, ind, v, dims = problem("helm", 64)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 128, is_complex=True, precision=precision)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+    if precision == "f64":
+        x = to_dev(b)
+        ft.solve(x)
+        eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
+        assert eta.max() <= 1e-13, eta
+    else:
+        x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=10)
+        eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
+        assert info["converged"] and eta.max() <= 1e-10, (info, eta)
+
+
+@pytest.mark.gpu
 def test_hermitian_ldlh_fp64(device):
     ip, ind, v, dims = hermitian_matrix(6)
     N = ip.size - 1
