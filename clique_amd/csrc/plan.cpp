@@ -728,20 +728,30 @@ struct Numeric : NumericBase {
 
     // Split K of tall, skinny products with a long K (the update-row products
     // of the solves, M x k with K = u or n): a launch with fewer than
-    // SPLIT_TILES 64x64 tiles leaves most CUs idle while each tile walks K in
-    // 16-deep steps, so every plain descriptor with K >= 2 SPLIT_MIN_K is cut
-    // into up to SPLIT_TARGET / tiles slices, written to ws and summed in
-    // slice order by a second launch. Returns the workspace entries used;
-    // with sb == nullptr nothing is scheduled.
-    static constexpr double SPLIT_TILES = 512.0, SPLIT_TARGET = 1024.0;
+    // split_tiles() 64x64 tiles (2,048; CLIQG_SPLIT_TILES) leaves CUs idle or
+    // unevenly loaded while each tile walks K in 16-deep steps, so every plain
+    // descriptor with K >= 2 SPLIT_MIN_K is cut into up to
+    // 2 split_tiles() / tiles slices, written to ws and summed in slice order
+    // by a second launch (measured on the 120^3 PML solves: no split 209 ms
+    // of solve products per five solves, 512: 200, 1,024: 195, 2,048: 192).
+    // Returns the workspace entries used; with sb == nullptr nothing is
+    // scheduled.
+    static double split_tiles() {
+        static const double v = [] {
+            const char* e = std::getenv("CLIQG_SPLIT_TILES");
+            const double t = e ? std::atof(e) : 2048.0;
+            return (t >= 0.0 && t <= 8192.0) ? t : 2048.0;
+        }();
+        return v;
+    }
     static constexpr int SPLIT_MIN_K = 512;
-    static int64_t split_bound() { return (int64_t)4096 * (int64_t)(SPLIT_TARGET + SPLIT_TILES); }
+    static int64_t split_bound() { return (int64_t)4096 * (int64_t)(3.0 * split_tiles() + 1.0); }
 
     int64_t split_gemm(StepBuilder<T>* sb, const std::vector<GemmDesc<T>>& g, T* ws) {
         double tiles = 0.0;
         for (const auto& d : g) tiles += gemm_tiles(d.M, d.N, d.flags, 0);
-        const int want = (tiles > 0.0 && tiles < SPLIT_TILES) ? (int)std::ceil(SPLIT_TARGET / tiles)
-                                                              : 1;
+        const double lim = split_tiles();
+        const int want = (tiles > 0.0 && tiles < lim) ? (int)std::ceil(2.0 * lim / tiles) : 1;
         std::vector<GemmDesc<T>> parts;
         std::vector<ReduceDesc<T>> red;
         int64_t off = 0;
