@@ -922,7 +922,7 @@ __global__ void k_solve_extend(const SolveExtend* __restrict__ ds, const int* __
     const int j0 = blockIdx.y * RHS_CHUNK;
     const int c = find_segment(row_start, nd, g);
     const SolveExtend d = ds[c];
-    const int i = g - row_start[c];
+    const int i = g - row_start[c] + d.i0;
     const T* wc = wpool + d.wc * k + i + (int64_t)j0 * d.uc;
     const int I = relpool[d.rel + i];
     T* dst;
@@ -987,14 +987,17 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
 // of columns, and for rows of at most 8 entries all of the thread's loads
 // are issued before its products.
 constexpr int SPMV_COLS = 8, SPMV_ROWNZ = 8;
-template <class V>
+// With xn, y is not written: row i of the result goes to xn(map[i], j) in the
+// working type (the nodal right-hand side of a refinement solve), ld N.
+template <class V, class T>
 __global__ __launch_bounds__(256) void k_spmv(int N, const int64_t* __restrict__ rowoff,
                                               const int* __restrict__ cols,
                                               const V* __restrict__ vals, double alpha,
                                               const V* __restrict__ x, int64_t ldx, double beta,
                                               const V* __restrict__ b, int64_t ldb,
                                               V* __restrict__ y, int64_t ldy, int k,
-                                              unsigned long long* colmax) {
+                                              unsigned long long* colmax, T* __restrict__ xn,
+                                              const int* __restrict__ map) {
     __shared__ double red[SPMV_COLS][4];
     // one-dimensional grid of (row block, column chunk) pairs, remapped so
     // each XCD runs a contiguous range of row blocks of one column chunk:
@@ -1047,7 +1050,8 @@ __global__ __launch_bounds__(256) void k_spmv(int N, const int64_t* __restrict__
             V* py = y + i + (int64_t)(j0 + q) * ldy;
             V out = scale(acc[q], alpha);
             if (beta != 0.0) out = out + scale(b ? b[i + (int64_t)(j0 + q) * ldb] : *py, beta);
-            *py = out;
+            if (xn) xn[map[i] + (int64_t)(j0 + q) * N] = cvt<T>(out);
+            else *py = out;
             m[q] = abs1(out);
         }
     }
@@ -1067,15 +1071,59 @@ __global__ __launch_bounds__(256) void k_spmv(int N, const int64_t* __restrict__
         }
     }
 }
+template <class V, class T>
+static void launch_spmv_any(int N, const int64_t* rowoff, const int* cols, const V* vals,
+                            double alpha, const V* x, int64_t ldx, double beta, const V* b,
+                            int64_t ldb, V* y, int64_t ldy, int k, unsigned long long* colmax,
+                            T* xn, const int* map, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    const int64_t blocks = (int64_t)((N + 255) / 256) * ((k + SPMV_COLS - 1) / SPMV_COLS);
+    if (blocks > 0x7fffffff) throw Error(1, "SpMV grid too large");
+    hipLaunchKernelGGL((k_spmv<V, T>), dim3((unsigned)blocks), dim3(256), 0, s, N, rowoff, cols,
+                       vals, alpha, x, ldx, beta, b, ldb, y, ldy, k, colmax, xn, map);
+}
 template <class V>
 void launch_spmv_ex(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
                     const V* x, int64_t ldx, double beta, const V* b, int64_t ldb, V* y,
                     int64_t ldy, int k, unsigned long long* colmax, hipStream_t s) {
+    launch_spmv_any<V, V>(N, rowoff, cols, vals, alpha, x, ldx, beta, b, ldb, y, ldy, k, colmax,
+                          nullptr, nullptr, s);
+}
+template <class T, class V>
+void launch_residual_nodal(int N, const int64_t* rowoff, const int* cols, const V* vals,
+                           const V* x, int64_t ldx, const V* b, int64_t ldb, int k,
+                           unsigned long long* colmax, T* xn, const int* map, hipStream_t s) {
+    launch_spmv_any<V, T>(N, rowoff, cols, vals, -1.0, x, ldx, 1.0, b, ldb, nullptr, 0, k, colmax,
+                          xn, map, s);
+}
+
+// x(i, j) += xn(map[i], j) with colmax[j] = max(colmax[j], max_i |x(i, j)|)
+template <class T, class V>
+__global__ __launch_bounds__(256) void k_push_add_abs(const T* __restrict__ xn,
+                                                      const int* __restrict__ map,
+                                                      V* __restrict__ x, int64_t ldx, int N,
+                                                      unsigned long long* colmax) {
+    __shared__ double red[4];
+    const T* xc = xn + (int64_t)blockIdx.y * N;
+    V* yc = x + (int64_t)blockIdx.y * ldx;
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const V v = yc[i] + cvt<V>(xc[map[i]]);
+        yc[i] = v;
+        m = fmax(m, abs1(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomic_max_nonneg(colmax + blockIdx.y, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
+}
+template <class T, class V>
+void launch_push_add_abs(const T* xn, const int* map, V* x, int64_t ldx, int N, int k,
+                         unsigned long long* colmax, hipStream_t s) {
     if (N <= 0 || k <= 0) return;
-    const int64_t blocks = (int64_t)((N + 255) / 256) * ((k + SPMV_COLS - 1) / SPMV_COLS);
-    if (blocks > 0x7fffffff) throw Error(1, "SpMV grid too large");
-    hipLaunchKernelGGL(k_spmv<V>, dim3((unsigned)blocks), dim3(256), 0, s, N, rowoff, cols, vals,
-                       alpha, x, ldx, beta, b, ldb, y, ldy, k, colmax);
+    hipLaunchKernelGGL((k_push_add_abs<T, V>), grid_cols(N, k), dim3(256), 0, s, xn, map, x, ldx,
+                       N, colmax);
 }
 template <class V>
 void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
@@ -1247,7 +1295,12 @@ CQ_INST_T(cdouble)
     template void launch_pull<T, V>(const V*, int64_t, const int*, T*, int, int, hipStream_t);   \
     template void launch_push<T, V>(const T*, const int*, V*, int64_t, int, int, hipStream_t);   \
     template void launch_diag_solve<T, V>(T*, const V*, int, int, int*, hipStream_t);           \
-    template void launch_widen<T, V>(const T*, V*, int64_t, hipStream_t);
+    template void launch_widen<T, V>(const T*, V*, int64_t, hipStream_t);                        \
+    template void launch_residual_nodal<T, V>(int, const int64_t*, const int*, const V*,         \
+                                              const V*, int64_t, const V*, int64_t, int,         \
+                                              unsigned long long*, T*, const int*, hipStream_t); \
+    template void launch_push_add_abs<T, V>(const T*, const int*, V*, int64_t, int, int,         \
+                                            unsigned long long*, hipStream_t);
 
 CQ_INST_TV(float, double)
 CQ_INST_TV(double, double)

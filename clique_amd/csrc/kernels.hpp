@@ -111,6 +111,7 @@ struct SolveExtend {
     int64_t wp;      // parent workspace offset (per right-hand side)
     int64_t rel;     // offset of the child's relative indices
     int uc, np, offp, up;
+    int i0;          // first child row of this launch (its count is in row_start)
 };
 
 // A contiguous range to clear.
@@ -194,6 +195,17 @@ template <class V>
 void launch_spmv_ex(int N, const int64_t* rowoff, const int* cols, const V* vals, double alpha,
                     const V* x, int64_t ldx, double beta, const V* b, int64_t ldb, V* y,
                     int64_t ldy, int k, unsigned long long* colmax, hipStream_t s);
+// xn(map[i], j) = b(i, j) - (A x)(i, j) in the working type, with
+// colmax[j] = max(colmax[j], max_i |b - A x|(i, j)) (a refinement step's
+// residual, handed to the solve in nodal order)
+template <class T, class V>
+void launch_residual_nodal(int N, const int64_t* rowoff, const int* cols, const V* vals,
+                           const V* x, int64_t ldx, const V* b, int64_t ldb, int k,
+                           unsigned long long* colmax, T* xn, const int* map, hipStream_t s);
+// x(i, j) += xn(map[i], j) with colmax[j] = max(colmax[j], max_i |x(i, j)|)
+template <class T, class V>
+void launch_push_add_abs(const T* xn, const int* map, V* x, int64_t ldx, int N, int k,
+                         unsigned long long* colmax, hipStream_t s);
 // y += x with colmax[j] = max(colmax[j], max_i |y(i, j)|)
 template <class V>
 void launch_axpy_abs(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k,

@@ -274,11 +274,12 @@ struct StepBuilder {
         // FP32: 128x64 with K = 128 8 us instead of 15, 14400x64 with
         // K = 128 16 instead of 20, 20000x64 with K = 7200 0.69 ms instead
         // of 1.12).
-        static const bool smallTiles = [] {
+        static const double smallLimit = [] {
             const char* e = std::getenv("CLIQG_SMALL_TILES");
-            return !e || std::atoi(e) != 0;
+            const double v = e ? std::atof(e) : SMALL_TILE_LIMIT;
+            return (v >= 0.0 && v <= 65536.0) ? v : SMALL_TILE_LIMIT;
         }();
-        if (smallTiles) {
+        if (smallLimit > 0.0) {
             // An in-place product (C is A or B, the TRSM of the factor) needs
             // every column of a row block in one tile: a second tile column
             // would read A after the first one has overwritten it.
@@ -288,7 +289,7 @@ struct StepBuilder {
                 tiles0 += gemm_tiles(d.M, d.N, d.flags, 0);
                 inplace = inplace || ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(5));
             }
-            if (!inplace && tiles0 > 0.0 && tiles0 <= SMALL_TILE_LIMIT) return 5;
+            if (!inplace && tiles0 > 0.0 && tiles0 <= smallLimit) return 5;
         }
         if (!gemm_variant_allowed<T>(4) || Scalar<T>::complex) return 0;
         const double rel4 = sizeof(T) == 4 ? 0.95 : 0.83;
@@ -845,20 +846,13 @@ struct Numeric : NumericBase {
                 const auto& lv = P.levels[h];
                 int maxn = 0;
                 for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
-                if (!bwd) {
-                    std::vector<ZeroSeg<T>> zs;
-                    std::vector<int> zst;
-                    int chunks = 0;
-                    for (int s : lv) {
-                        const int64_t n = (int64_t)P.fronts[s].u * k;
-                        if (n == 0) continue;
-                        zs.push_back(ZeroSeg<T>{Wp(s), n});
-                        zst.push_back(chunks);
-                        chunks += (int)((n + ZERO_CHUNK - 1) / ZERO_CHUNK);
-                    }
-                    if (!zs.empty())
-                        steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
-                                             (int)zs.size(), chunks, false});
+                // Forward extend-add in two passes (lower_solve/local.hpp:86-104
+                // adds a child's update rows into the parent in one): the rows
+                // that land in the parent's own rows (a prefix: the relative
+                // indices increase) before the parent's panels, the rest after
+                // its update-row product, which then writes W_s instead of
+                // accumulating into a cleared one.
+                auto sextend = [&](bool wpart) {
                     for (int slot = 0; slot < 2; ++slot) {
                         std::vector<SolveExtend> ds;
                         std::vector<int> st;
@@ -868,14 +862,36 @@ struct Numeric : NumericBase {
                             if (fp.nchild <= slot) continue;
                             const FrontLayout& fc = P.fronts[fp.child[slot]];
                             if (fc.u == 0) continue;
-                            ds.push_back(SolveExtend{fc.W, fp.W, fc.rel, fc.u, fp.n, fp.off, fp.u});
+                            const int* r = P.relHost.data() + fc.rel;
+                            const int is = (int)(std::lower_bound(r, r + fc.u, fp.n) - r);
+                            const int i0 = wpart ? is : 0, nr = wpart ? fc.u - is : is;
+                            if (nr == 0) continue;
+                            ds.push_back(SolveExtend{fc.W, fp.W, fc.rel, fc.u, fp.n, fp.off, fp.u,
+                                                     i0});
                             st.push_back(tot);
-                            tot += fc.u;
+                            tot += nr;
                         }
                         if (!ds.empty())
                             steps.push_back(Step{S_SEXTEND, S.arena.add(ds), S.arena.add(st),
                                                  (int)ds.size(), tot, false});
                     }
+                };
+                if (!bwd) {
+                    // W_s of fronts without columns gets no update-row product
+                    std::vector<ZeroSeg<T>> zs;
+                    std::vector<int> zst;
+                    int chunks = 0;
+                    for (int s : lv) {
+                        const int64_t n = (int64_t)P.fronts[s].u * k;
+                        if (n == 0 || P.fronts[s].n > 0) continue;
+                        zs.push_back(ZeroSeg<T>{Wp(s), n});
+                        zst.push_back(chunks);
+                        chunks += (int)((n + ZERO_CHUNK - 1) / ZERO_CHUNK);
+                    }
+                    if (!zs.empty())
+                        steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
+                                             (int)zs.size(), chunks, false});
+                    sextend(false);
                     if (blk) {
                         // FrontBlockLowerForwardSolve (lower_solve/local_front_block.hpp:28-48):
                         // X_s := inv(ATL) X_s through a copy of X_s
@@ -917,9 +933,10 @@ struct Numeric : NumericBase {
                         const FrontLayout& f = P.fronts[s];
                         if (f.u == 0 || f.n == 0) continue;
                         g.push_back(GemmDesc<T>{L(s) + f.n, X2(s), Wp(s), nullptr, nullptr, f.m, N,
-                                                f.u, f.u, k, f.n, G_ACCUM | G_NEG});
+                                                f.u, f.u, k, f.n, G_NEG});
                     }
                     split_gemm(&sb, g, ws);
+                    sextend(true);
                 } else {
                     std::vector<SolveSeg> segs;
                     std::vector<int> rs;
@@ -1137,18 +1154,23 @@ struct Numeric : NumericBase {
             map = identity.as<int>();
         }
         launch_pull<T, V>(x, ldx, map, S.xn.template as<T>(), P.N, k, st);
+        solve_core(S, k);
+        launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
+        CQ_HIP(hipGetLastError());
+    }
+
+    // forward, diagonal and backward sweeps on the nodal buffer S.xn
+    void solve_core(SolveSched& S, int k) {
         auto body = [&] {
             run(S.fwd, S.arena, &S, k);
             // block fronts hold inv(ATL), so there is no separate diagonal
             // solve (solve.hpp:48-53)
             if (!block())
                 launch_diag_solve<T, V>(S.xn2.template as<T>(), dpool.as<V>(), P.N, k,
-                                        P.dStatus.as<int>(), st);
+                                        P.dStatus.as<int>(), P.stream);
             run(S.bwd, S.arena, &S, k);
         };
         if (!replay(S.graph, body)) body();
-        launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
-        CQ_HIP(hipGetLastError());
     }
 
     V* scratch_cols(int k) {
@@ -1479,7 +1501,6 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
         auto* nm = static_cast<Numeric<T>*>(num.get());
         const V* b = static_cast<const V*>(d_b);
         V* x = static_cast<V*>(d_x);
-        V* r = nm->scratch_cols(k);
         unsigned long long* nrm = dNorms.as<unsigned long long>();
         // nrm[0] = ||A||, nrm[1..k] = |b|, [k+1..2k] = |x|, [2k+1..3k] = |r|
         CQ_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long) * 3 * k, stream));
@@ -1491,10 +1512,16 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
         int it = 0;
         bool conv = false;
         double eta = 0.0;
+        if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
+        auto& S = nm->schedule(k);
+        T* xnb = S.xn.template as<T>();
         for (;;) {
-            // r = b - A x with |r| per column in one pass; |x| came with x
-            launch_spmv_ex<V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<V>(), -1.0, x,
-                              ldx, 1.0, b, ldb, r, N, k, nrm + 1 + 2 * k, stream);
+            // r = b - A x with |r| per column in one pass, written in nodal
+            // order and working precision straight into the solve's buffer;
+            // |x| came with x
+            launch_residual_nodal<T, V>(N, dRowOff.as<int64_t>(), dCols.as<int>(), dVals.as<V>(), x,
+                                        ldx, b, ldb, k, nrm + 1 + 2 * k, xnb, dMap.as<int>(),
+                                        stream);
             CQ_HIP(hipMemcpyAsync(h.data(), nrm, sizeof(unsigned long long) * (3 * k + 1),
                                   hipMemcpyDeviceToHost, stream));
             CQ_HIP(hipStreamSynchronize(stream));
@@ -1510,9 +1537,9 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
             }
             if (eta <= tol) { conv = true; break; }
             if (it >= maxIters) break;
-            nm->solve(r, N, k, false);
+            nm->solve_core(S, k);
             CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
-            launch_axpy_abs<V>(x, ldx, r, N, N, k, nrm + 1 + k, stream);
+            launch_push_add_abs<T, V>(xnb, dMap.as<int>(), x, ldx, N, k, nrm + 1 + k, stream);
             ++it;
         }
         *itersOut = it;
