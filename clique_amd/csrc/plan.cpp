@@ -1504,17 +1504,19 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
         unsigned long long* nrm = dNorms.as<unsigned long long>();
         // nrm[0] = ||A||, nrm[1..k] = |b|, [k+1..2k] = |x|, [2k+1..3k] = |r|
         CQ_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long) * 3 * k, stream));
+        if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
+        auto& S = nm->schedule(k);
+        T* xnb = S.xn.template as<T>();
+        // x0 = solve(b): b goes straight into the solve's nodal buffer
         launch_col_abs_max<V>(b, ldb, N, k, nrm + 1, stream);
-        launch_copy_cols<V>(x, ldx, b, ldb, N, k, stream);
-        nm->solve(x, ldx, k, false);
+        launch_pull<T, V>(b, ldb, dMap.as<int>(), xnb, N, k, stream);
+        nm->solve_core(S, k);
+        launch_push<T, V>(xnb, dMap.as<int>(), x, ldx, N, k, stream);
         launch_col_abs_max<V>(x, ldx, N, k, nrm + 1 + k, stream);
         std::vector<unsigned long long> h(3 * k + 1);
         int it = 0;
         bool conv = false;
         double eta = 0.0;
-        if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
-        auto& S = nm->schedule(k);
-        T* xnb = S.xn.template as<T>();
         for (;;) {
             // r = b - A x with |r| per column in one pass, written in nodal
             // order and working precision straight into the solve's buffer;
