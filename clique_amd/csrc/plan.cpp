@@ -1257,6 +1257,10 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
             f.slot = (p.children.size() > 0 && p.children[0] == s) ? 0 : 1;
             const std::vector<int>& r = f.slot == 0 ? p.leftRelInds : p.rightRelInds;
             if ((int)r.size() != f.u) throw Error(1, "relative indices do not match the update size");
+            // the extend-adds split a child's update at its first index past
+            // the parent's own columns, which needs increasing relative indices
+            for (size_t t = 1; t < r.size(); ++t)
+                if (r[t] <= r[t - 1]) throw Error(1, "relative indices are not increasing");
             relAll.insert(relAll.end(), r.begin(), r.end());
         } else if (f.u != 0) {
             throw Error(1, "a root front has a nonempty lower structure");

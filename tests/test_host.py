@@ -106,3 +106,22 @@ def test_no_cpu_fallback_in_product():
             if fn.endswith((".py", ".cpp", ".hip", ".hpp")):
                 txt = open(os.path.join(dirpath, fn)).read()
                 assert "import oracle" not in txt and "liboracle" not in txt, fn
+
+
+@pytest.mark.parametrize("dims,cut", [((6, 6, 6), 8), ((30, 30, 30), 128), ((12, 9, 1), 10),
+                                      ((11, 3, 2), 1)])
+def test_relative_indices_increase(dims, cut):
+    # the two-pass extend-adds (factor and forward solve) split each child's
+    # update at its first index past the parent's own columns: the relative
+    # indices of every child must increase, the own-column ones first
+    ip, ind, v, d = P.stencil7(*dims, 6.0)
+    s = cq.NaturalNestedDissection(*d, ip, ind, cutoff=cut)
+    for k in range(s.num_nodes):
+        nd = s.node(k)
+        for w in (4, 5):
+            rel = s.node_array(k, w)
+            assert np.all(np.diff(rel) > 0), (k, w)
+            if rel.size:
+                own = rel < nd["size"]
+                # a prefix of own-column indices, then the update rows
+                assert not np.any(own[1:] & ~own[:-1]), (k, w)
