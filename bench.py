@@ -177,8 +177,8 @@ def main():
                     help="two FP32 factorizations and a calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the multifrontal factor and solves run on "
+              "one MI355X here, and runs on several GPUs are not implemented", file=sys.stderr)
         sys.exit(2)
     if args.factor_only:
         factor_only(args)

@@ -83,8 +83,9 @@ def test_natural_dissection_closed_forms():
             assert np.all(z == (nz - 1) // 2)
 
 
-def test_survey_front_counts():
-    """Tree sizes quoted for the benchmark configurations."""
+def test_benchmark_tree_sizes():
+    """Front counts and FP64 factor work of the 30^3 and 64^3 benchmark trees
+    (363 fronts and 4.365 GFLOP; 4,095 fronts and 1,696.9 GFLOP complex)."""
     for n, fronts_, gflop in ((30, 363, 4.365), (64, 4095, 1696.9 / 4)):
         ip, ind, v, dims = P.stencil7(n, n, n, 6.0)
         t = oracle.Tree(*dims, ip, ind, cutoff=128)
