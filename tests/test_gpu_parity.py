@@ -404,3 +404,39 @@ def test_grf_diffusion_matches_oracle(device, n, cut):
     xr, info = ft32.refine(to_dev(b), tol=1e-10, max_iters=30)
     assert info["converged"]
     assert backward_error(ip, ind, v, xr.cpu().numpy(), b).max() <= 1e-10
+
+
+_SWITCH_SCRIPT = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+import clique_amd as cq
+from helpers import backward_error, problem
+ip, ind, v, dims = problem("pml", 40, pml=5)
+N = ip.size - 1
+sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
+ft = cq.DistSymmFrontTree(sym, ip, ind, is_complex=True)
+ft.factor(torch.tensor(v, device="cuda:0"), precision="f64")
+rng = np.random.default_rng(7)
+b = rng.standard_normal((N, 4)) + 1j * rng.standard_normal((N, 4))
+x = cq.colmajor(torch.tensor(b, device="cuda:0"))
+ft.solve(x)
+print("%.3e" % backward_error(ip, ind, v, x.cpu().numpy(), b).max())
+"""
+
+
+@pytest.mark.gpu
+def test_tuning_switches_keep_results(device):
+    # the non-default schedule paths (no small tiles, no split K, no
+    # lookahead, 128-wide outer panels, 64-wide solve blocks) solve to the
+    # same FP64 rounding level; the switches are read once per process
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, CLIQG_SMALL_TILES="0", CLIQG_SPLIT_TILES="0", CLIQG_LOOKAHEAD="0",
+               CLIQG_OUTER_PANEL="128", CLIQG_SOLVE_BLOCK="64")
+    r = subprocess.run([sys.executable, "-c", _SWITCH_SCRIPT, root], env=env,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert float(r.stdout.strip().splitlines()[-1]) <= 1e-13, r.stdout
