@@ -657,30 +657,50 @@ void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bo
 }
 
 // ---------------------------------------------------------------------------
-// Extend-add: one wave per (child, update column), four per workgroup (most
-// update columns are short: a workgroup each would make the launch
-// dispatch-bound). Children of one slot have distinct parents, so no two
-// waves touch the same entry.
+// Extend-add: one wave per pair of child update columns (j0 + p and
+// j0 + nc - 1 - p, so every wave adds about the same number of entries),
+// four waves per workgroup. Children of one slot have distinct parents, so no
+// two waves touch the same entry. Each lane issues the loads of four rows
+// before their stores (the pools may alias, so the compiler would not).
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restrict__ descs,
-                                                    const int* __restrict__ col_start, int nd,
-                                                    int ncols) {
-    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= ncols) return;
-    const int lane = threadIdx.x & 63;
-    const int c = find_segment(col_start, nd, g);
-    const ExtendDesc<T> d = descs[c];
-    const int j = g - col_start[c] + d.j0;
+__device__ __forceinline__ void extend_column(const ExtendDesc<T>& d, int j, int lane) {
     const int J = d.rel[j];
     const T* col = d.Uc + (int64_t)j * d.uc;
     T* dst;
     if (J < d.np) dst = d.Lp + (int64_t)J * d.mp;
     else dst = d.Up + (int64_t)(J - d.np) * d.up - d.np;
-    for (int i = j + lane; i < d.uc; i += 64) {
-        T* p = dst + d.rel[i];
-        *p = *p + col[i];
+    for (int i = j + lane; i < d.uc; i += 256) {
+        int r[4];
+        T v[4], w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i + 64 * q < d.uc) {
+                r[q] = d.rel[i + 64 * q];
+                v[q] = col[i + 64 * q];
+            }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i + 64 * q < d.uc) w[q] = dst[r[q]];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i + 64 * q < d.uc) dst[r[q]] = w[q] + v[q];
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_extend_add(const ExtendDesc<T>* __restrict__ descs,
+                                                    const int* __restrict__ pair_start, int nd,
+                                                    int npairs) {
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= npairs) return;
+    const int lane = threadIdx.x & 63;
+    const int c = find_segment(pair_start, nd, g);
+    const ExtendDesc<T> d = descs[c];
+    const int p = g - pair_start[c];
+    const int j = d.j0 + p, j2 = d.j0 + d.nc - 1 - p;
+    extend_column(d, j, lane);
+    if (j2 != j) extend_column(d, j2, lane);
 }
 
 template <class T>

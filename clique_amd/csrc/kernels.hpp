@@ -79,7 +79,7 @@ struct ExtendDesc {
     T* Lp;            // parent frontL (mp x np, ld mp)
     T* Up;            // parent update matrix (up x up, ld up)
     int uc, np, mp, up;
-    int j0;           // first child column of this launch (its count is in col_start)
+    int j0, nc;       // child columns j0..j0+nc-1 (col_start counts column pairs)
 };
 
 // One front of a diagonal-block factorization step.

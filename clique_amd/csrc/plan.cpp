@@ -472,9 +472,9 @@ struct Numeric : NumericBase {
                         const int j0 = upart ? js : 0, nc = upart ? fc.u - js : js;
                         if (nc == 0) continue;
                         ds.push_back(ExtendDesc<T>{U(c), P.dRel.as<int>() + fc.rel, L(p), U(p),
-                                                   fc.u, fp.n, fp.m, fp.u, j0});
+                                                   fc.u, fp.n, fp.m, fp.u, j0, nc});
                         start.push_back(total);
-                        total += nc;
+                        total += (nc + 1) / 2;   // column pairs
                         // entries of columns j0..j0+nc-1 of a lower triangle of order uc
                         const double e = (double)nc * fc.u - ((double)(j0 + nc) * (j0 + nc - 1) -
                                                                (double)j0 * (j0 - 1)) / 2.0;
