@@ -173,6 +173,7 @@ class DistSymmFrontTree:
         self._h = h
         self.front_type = None
         self.precision = None
+        self.factor_count = 0   # numeric factorizations so far (refactoring changes it)
         _LIVE.add(self)
 
     def close(self):
@@ -199,6 +200,7 @@ class DistSymmFrontTree:
         self.precision = precision
         self.front_type = front_type
         self._vals = vals
+        self.factor_count += 1
 
     def solve(self, x, nodal=False):
         t, ld, k = _check_rhs(x, self.N, self.is_complex)

@@ -292,6 +292,15 @@ cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* 
     return guarded([&] {
         auto& p = P(plan);
         require(ldb >= p.N && ldx >= p.N && k >= 0 && max_iters >= 0, "bad arguments");
+        if (k > 0) {
+            // x is written before every residual reads b: the two ranges must not overlap
+            const size_t vb = p.complex ? 16 : 8;
+            const char* b0 = static_cast<const char*>(d_b);
+            const char* x0 = static_cast<const char*>(d_x);
+            const char* b1 = b0 + ((size_t)ldb * (k - 1) + p.N) * vb;
+            const char* x1 = x0 + ((size_t)ldx * (k - 1) + p.N) * vb;
+            require(b1 <= x0 || x1 <= b0, "refinement needs x and b in distinct device memory");
+        }
         int it = 0, conv = 0;
         double eta = 0.0;
         p.refine(d_b, ldb, d_x, ldx, k, tol, max_iters, &it, &conv, &eta);

@@ -164,8 +164,8 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 
 }  // namespace
 
-int gemm_tile_m(int variant) { return variant == 5 ? 32 : (variant == 0 ? 64 : 128); }
-int gemm_tile_n(int variant) { return variant == 5 ? 32 : ((variant >= 2) ? 128 : 64); }
+int gemm_tile_m(int variant) { return variant == TILE_32 ? 32 : (variant == TILE_128 ? 128 : 64); }
+int gemm_tile_n(int variant) { return gemm_tile_m(variant); }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
@@ -508,26 +508,13 @@ template <class T>
 void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc, int ntiles,
                  int variant, int role, hipStream_t s) {
     if (ntiles <= 0 || ndesc <= 0) return;
-    if (variant == 2) {
-        if constexpr (gemm_variant_allowed<T>(2))
-            launch_gemm_shape<T, 128, 128, 16, 2, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
-        else
-            throw Error(1, "128x128 GEMM tiles are not built for complex double");
-    } else if (variant == 1) {
-        launch_gemm_shape<T, 128, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
-    } else if (variant == 3) {
-        if constexpr (gemm_variant_allowed<T>(3))
-            launch_gemm_shape<T, 128, 128, 16, 4, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
-        else
-            throw Error(1, "128x128 GEMM tiles are not built for complex double");
-    } else if (variant == 4) {
-        if constexpr (gemm_variant_allowed<T>(4))
+    if (variant == TILE_128) {
+        if constexpr (gemm_variant_allowed<T>(TILE_128))
             launch_gemm_shape<T, 128, 128, 16, 2, 2>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
-            throw Error(1, "128x128 GEMM tiles are not built for complex double");
-    } else if (variant == 5) {
+            throw Error(1, "128x128 GEMM tiles are built for real types only");
+    } else if (variant == TILE_32) {
         launch_gemm_shape<T, 32, 32>(d_descs, d_tile_start, ndesc, ntiles, role, s);
-
     } else {
         launch_gemm_shape<T, 64, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     }
@@ -1152,33 +1139,6 @@ void launch_spmv(int N, const int64_t* rowoff, const int* cols, const V* vals, d
                       s);
 }
 
-// y += x column by column, with colmax[j] = max(colmax[j], max_i |y(i, j)|)
-template <class V>
-__global__ __launch_bounds__(256) void k_axpy_abs(V* __restrict__ y, int64_t ldy,
-                                                  const V* __restrict__ x, int64_t ldx, int N,
-                                                  unsigned long long* colmax) {
-    __shared__ double red[4];
-    V* yc = y + (int64_t)blockIdx.y * ldy;
-    const V* xc = x + (int64_t)blockIdx.y * ldx;
-    double m = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
-        const V v = yc[i] + xc[i];
-        yc[i] = v;
-        m = fmax(m, abs1(v));
-    }
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        atomic_max_nonneg(colmax + blockIdx.y, fmax(fmax(red[0], red[1]), fmax(red[2], red[3])));
-}
-template <class V>
-void launch_axpy_abs(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k,
-                     unsigned long long* colmax, hipStream_t s) {
-    if (N <= 0 || k <= 0) return;
-    hipLaunchKernelGGL(k_axpy_abs<V>, grid_cols(N, k), dim3(256), 0, s, y, ldy, x, ldx, N, colmax);
-}
-
 template <class V>
 __global__ void k_row_abs_sum_max(int N, const int64_t* __restrict__ rowoff,
                                   const V* __restrict__ vals, unsigned long long* out) {
@@ -1226,20 +1186,6 @@ void launch_col_abs_max(const V* x, int64_t ldx, int N, int k, unsigned long lon
     unsigned gx = grid_for(N);
     if (gx > 256) gx = 256;
     hipLaunchKernelGGL(k_col_abs_max<V>, dim3(gx, k), dim3(256), 0, s, x, ldx, N, out);
-}
-
-template <class V>
-__global__ void k_axpy_cols(V* __restrict__ y, int64_t ldy, const V* __restrict__ x, int64_t ldx,
-                            int N) {
-    V* yc = y + (int64_t)blockIdx.y * ldy;
-    const V* xc = x + (int64_t)blockIdx.y * ldx;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        yc[i] = yc[i] + xc[i];
-}
-template <class V>
-void launch_axpy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s) {
-    if (N <= 0 || k <= 0) return;
-    hipLaunchKernelGGL(k_axpy_cols<V>, grid_cols(N, k), dim3(256), 0, s, y, ldy, x, ldx, N);
 }
 
 template <class V>
@@ -1333,13 +1279,10 @@ CQ_INST_TV(cdouble, cdouble)
     template void launch_spmv_ex<V>(int, const int64_t*, const int*, const V*, double,          \
                                     const V*, int64_t, double, const V*, int64_t, V*, int64_t,  \
                                     int, unsigned long long*, hipStream_t);                    \
-    template void launch_axpy_abs<V>(V*, int64_t, const V*, int64_t, int, int,                   \
-                                     unsigned long long*, hipStream_t);                        \
     template void launch_row_abs_sum_max<V>(int, const int64_t*, const V*, unsigned long long*, \
                                             hipStream_t);                                      \
     template void launch_col_abs_max<V>(const V*, int64_t, int, int, unsigned long long*,       \
                                         hipStream_t);                                          \
-    template void launch_axpy_cols<V>(V*, int64_t, const V*, int64_t, int, int, hipStream_t);   \
     template void launch_copy_cols<V>(V*, int64_t, const V*, int64_t, int, int, hipStream_t);   \
     template void launch_adjoint_dA<V>(int, const int64_t*, const int*, const V*, int64_t,      \
                                        const V*, int64_t, int, bool, V*, hipStream_t);

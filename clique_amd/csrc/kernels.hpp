@@ -10,15 +10,20 @@ namespace cliqg {
 
 // Width of a diagonal block (panel) of the partial LDL^T.
 constexpr int NB = 64;
-// Tile shapes of the grouped GEMM: variant 0 = 64x64 (4 waves), 1 = 128x64
-// (4 waves), 2 = 128x128 (8 waves), 3 = 128x128 (16 waves), 4 = 128x128
-// (4 waves of 64x64), 5 = 32x32 (4 waves of 16x16, for latency-bound
-// launches); 2-4 are not built for complex double (LDS).
+// Tile shapes of the grouped GEMM (every launch uses one of them; the value
+// is the `variant` of a launch and appears in profile traces).
+enum GemmTile : int {
+    TILE_64 = 0,    // 64x64 tiles, four waves of 32x32 (the default)
+    TILE_128 = 4,   // 128x128 tiles, four waves of 64x64 (real types, large launches)
+    TILE_32 = 5,    // 32x32 tiles, four waves of 16x16 (latency-bound launches)
+};
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
+// 128x128 tiles are built for the real types only: complex FP32 keeps its
+// three-multiplication 64x64 tiles, complex FP64 would exceed the LDS.
 template <class T>
 constexpr bool gemm_variant_allowed(int variant) {
-    return variant < 2 || variant == 5 || !(Scalar<T>::complex && sizeof(T) == 16);
+    return variant != TILE_128 || !Scalar<T>::complex;
 }
 
 enum GemmFlags : int {
@@ -206,18 +211,12 @@ void launch_residual_nodal(int N, const int64_t* rowoff, const int* cols, const 
 template <class T, class V>
 void launch_push_add_abs(const T* xn, const int* map, V* x, int64_t ldx, int N, int k,
                          unsigned long long* colmax, hipStream_t s);
-// y += x with colmax[j] = max(colmax[j], max_i |y(i, j)|)
-template <class V>
-void launch_axpy_abs(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k,
-                     unsigned long long* colmax, hipStream_t s);
 template <class V>
 void launch_row_abs_sum_max(int N, const int64_t* rowoff, const V* vals, unsigned long long* out,
                             hipStream_t s);
 template <class V>
 void launch_col_abs_max(const V* x, int64_t ldx, int N, int k, unsigned long long* out,
                         hipStream_t s);
-template <class V>
-void launch_axpy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s);
 template <class V>
 void launch_copy_cols(V* y, int64_t ldy, const V* x, int64_t ldx, int N, int k, hipStream_t s);
 template <class V>

@@ -223,13 +223,12 @@ struct StepBuilder {
         if (steps.size() > n0) steps.back().update = true;
     }
 
-    // variant >= 0 forces a tile shape.
-    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false, int forced = -1) {
+    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) {
         std::vector<GemmDesc<T>> ds;
         std::vector<int> start;
         int total = 0;
         double flops = 0.0, bytes = 0.0;
-        const int variant = forced >= 0 ? forced : choose_variant(in);
+        const int variant = choose_variant(in);
         for (const auto& d : in)
             if ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(variant))
                 throw Error(1, "an in-place GEMM needs all of its columns in one tile");
@@ -286,15 +285,15 @@ struct StepBuilder {
             double tiles0 = 0.0;
             bool inplace = false;
             for (const auto& d : in) {
-                tiles0 += gemm_tiles(d.M, d.N, d.flags, 0);
-                inplace = inplace || ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(5));
+                tiles0 += gemm_tiles(d.M, d.N, d.flags, TILE_64);
+                inplace = inplace || ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(TILE_32));
             }
-            if (!inplace && tiles0 > 0.0 && tiles0 <= smallLimit) return 5;
+            if (!inplace && tiles0 > 0.0 && tiles0 <= smallLimit) return TILE_32;
         }
-        if (!gemm_variant_allowed<T>(4) || Scalar<T>::complex) return 0;
+        if (!gemm_variant_allowed<T>(TILE_128)) return TILE_64;
         const double rel4 = sizeof(T) == 4 ? 0.95 : 0.83;
         double cost[2] = {0.0, 0.0};
-        const int vs[2] = {0, 4};
+        const int vs[2] = {TILE_64, TILE_128};
         for (int c = 0; c < 2; ++c) {
             const int v = vs[c];
             double tiles = 0.0, work = 0.0;
@@ -303,12 +302,12 @@ struct StepBuilder {
                 tiles += t;
                 work += (double)t * std::max(d.K, 1);
             }
-            if (tiles == 0) return 0;
+            if (tiles == 0) return TILE_64;
             // average K per tile times the tile area, times rounds over the CUs
             const double rounds = std::ceil(tiles / 256.0);
             cost[c] = (c ? rel4 : 1.0) * rounds * (work / tiles) * gemm_tile_m(v) * gemm_tile_n(v);
         }
-        return cost[1] < cost[0] ? 4 : 0;
+        return cost[1] < cost[0] ? TILE_128 : TILE_64;
     }
 };
 
@@ -401,7 +400,7 @@ struct Numeric : NumericBase {
                    identity.bytes + linvw.bytes + wtmp.bytes + wbuf.bytes;
         for (auto& kv : solves)
             b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
-                 kv.second->tmp.bytes;
+                 kv.second->xn2.bytes + kv.second->tmp.bytes + kv.second->splitWs.bytes;
         return b;
     }
 
@@ -750,7 +749,7 @@ struct Numeric : NumericBase {
 
     int64_t split_gemm(StepBuilder<T>* sb, const std::vector<GemmDesc<T>>& g, T* ws) {
         double tiles = 0.0;
-        for (const auto& d : g) tiles += gemm_tiles(d.M, d.N, d.flags, 0);
+        for (const auto& d : g) tiles += gemm_tiles(d.M, d.N, d.flags, TILE_64);
         const double lim = split_tiles();
         const int want = (tiles > 0.0 && tiles < lim) ? (int)std::ceil(2.0 * lim / tiles) : 1;
         std::vector<GemmDesc<T>> parts;
@@ -1070,7 +1069,7 @@ struct Numeric : NumericBase {
                 // (32x32-tile updates apart: a different kernel)
                 const char* name = (s.kind == S_GEMM && !S)
                                        ? ((s.update || s.scaled)
-                                              ? (s.variant == 5 ? "gemm_update_small" : "gemm_update")
+                                              ? (s.variant == TILE_32 ? "gemm_update_small" : "gemm_update")
                                               : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
                                                              : step_name(kind);

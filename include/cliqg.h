@@ -159,7 +159,8 @@ cliqg_status cliqg_spmv(int N, const int64_t* d_row_off, const int* d_cols, cons
 
 /* FP64 iterative refinement (no reference counterpart): x := solve(b); while
  * some column has backward error > tol and iterations < max_iters:
- * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified. */
+ * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified;
+ * x and b must not overlap (CLIQG_ERR_INVALID otherwise). */
 cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* d_x, int ldx,
                                int k, double tol, int max_iters, cliqg_refine_info* info);
 
