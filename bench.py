@@ -88,6 +88,22 @@ def workload(cfg, n_override=None):
     return ip, ind, v, dims, b, True, desc
 
 
+def symbolic_for(ip, ind, dims, cutoff=128):
+    """Natural nested dissection of a workload; a list of grid dimensions is
+    a forest of independent problems (one symbolic per block, concatenated)."""
+    import clique_amd as cq
+    if not isinstance(dims, list):
+        return cq.NaturalNestedDissection(*dims, ip, ind, cutoff=cutoff)
+    syms, o = [], 0
+    for d in dims:
+        n_i = int(np.prod(d))
+        sub_ip = ip[o:o + n_i + 1] - ip[o]
+        sub_ind = ind[ip[o]:ip[o + n_i]] - o
+        syms.append(cq.NaturalNestedDissection(*d, sub_ip, sub_ind, cutoff=cutoff))
+        o += n_i
+    return cq.Symbolic.concat(syms)
+
+
 def cpu_baseline(budget_s=20.0):
     """Oracle FP64 LDL (the reference's CPU algorithm restated in C, OpenMP)
     on a bounded sample of the same operator family; TFLOP/s."""
@@ -191,18 +207,7 @@ def main():
     ip, ind, v, dims, b, cplx, desc = workload(args.config, args.n)
     N = ip.size - 1
     k = b.shape[1]
-    if isinstance(dims, list):
-        # a forest of independent problems: one symbolic per block, concatenated
-        syms, o, e = [], 0, 0
-        for d in dims:
-            n_i = int(np.prod(d))
-            sub_ip = ip[o:o + n_i + 1] - ip[o]
-            sub_ind = ind[ip[o]:ip[o + n_i]] - o
-            syms.append(cq.NaturalNestedDissection(*d, sub_ip, sub_ind, cutoff=128))
-            o += n_i
-        sym = cq.Symbolic.concat(syms)
-    else:
-        sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
+    sym = symbolic_for(ip, ind, dims)
     flops, solve_flops, entries = sym.work(cplx)
     dev = torch.device("cuda:0")
     vals = torch.tensor(v, device=dev)

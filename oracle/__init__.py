@@ -49,6 +49,7 @@ def lib():
         L.or_initialize.argtypes = [vp, ctypes.c_int, lp, ip, dp]
         L.or_ldl.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_solve.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int]
+        L.or_plant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.or_get_front.argtypes = [vp, ctypes.c_int, dp, dp]
         L.or_multiply.argtypes = [ctypes.c_int, lp, ip, dp, ctypes.c_int, ctypes.c_double,
                                   dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
@@ -150,6 +151,11 @@ class Factor:
         r = lib().or_ldl(self._h, nb, int(block), nthreads)
         if r != 0:
             raise ZeroDivisionError(f"oracle LDL hit a zero pivot ({r})")
+
+    def plant(self, front, panel):
+        """Test hook: the next ldl() skips the trailing update of one panel
+        of one front (a deliberate mistake the tolerances must catch)."""
+        lib().or_plant(self._h, int(front), int(panel))
 
     def solve_nodal(self, x):
         """Solve A_nodal y = x for x in nodal ordering (N or N x k)."""

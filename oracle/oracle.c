@@ -333,6 +333,7 @@ void or_factor_work(const or_tree* t, int is_complex, double* ff, double* sf, do
 struct or_factor {
     const or_tree* t;
     int is_complex, conjugate, block, factored;
+    int plant_front, plant_panel;   /* planted mistake of the next LDL (-1: none) */
     void** L;      /* per front m x n */
     void** diag;   /* per front n     */
 };
@@ -358,6 +359,7 @@ or_factor* or_factor_create(const or_tree* t, int is_complex, int conjugate) {
     f->t = t;
     f->is_complex = is_complex;
     f->conjugate = conjugate;
+    f->plant_front = f->plant_panel = -1;
     f->L = (void**)calloc((size_t)t->num, sizeof(void*));
     f->diag = (void**)calloc((size_t)t->num, sizeof(void*));
     return f;
@@ -384,6 +386,11 @@ int or_ldl(or_factor* f, int nb, int block, int nthreads) {
     int r = f->is_complex ? ldl_z(f, nb, block) : ldl_d(f, nb, block);
     if (r == 0) f->factored = 1;
     return r;
+}
+
+void or_plant(or_factor* f, int front, int panel) {
+    f->plant_front = front;
+    f->plant_panel = panel;
 }
 
 int or_solve(or_factor* f, double* x, int ldx, int k) {

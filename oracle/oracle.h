@@ -64,6 +64,10 @@ int or_initialize(or_factor* f, int N, const long* rowoff, const int* cols,
                   const double* vals);
 /* LocalLDL; block != 0 selects FrontBlockLDL; nthreads <= 0: default */
 int or_ldl(or_factor* f, int nb, int block, int nthreads);
+/* Test hook: the next or_ldl leaves out the trailing update of panel
+ * `panel` of front `front` (front < 0 clears it). Used only to show that the
+ * tests' tolerances detect such a mistake. */
+void or_plant(or_factor* f, int front, int panel);
 /* Solve in nodal ordering, X column-major N x k with leading dimension ldx */
 int or_solve(or_factor* f, double* x, int ldx, int k);
 /* copy front s: L is m x n column-major (ld = m), diag n entries */

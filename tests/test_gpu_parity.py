@@ -35,6 +35,11 @@ def to_dev(a):
     return cq.colmajor(torch.tensor(np.asarray(a), device="cuda:0"))
 
 
+# FP32 factor + FP64 refinement: every workload of this file converges to
+# 1e-10 in at most one step on MI355X; the cap allows two more.
+REFINE_CAP = 3
+
+
 CASES = [
     ("box", (2, 1, 1), 1),
     ("box", (7, 5, 3), 4),
@@ -137,7 +142,7 @@ def test_helmholtz64_split_k_solves(device, precision):
         eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
         assert eta.max() <= 1e-13, eta
     else:
-        x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=10)
+        x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=REFINE_CAP)
         eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
         assert info["converged"] and eta.max() <= 1e-10, (info, eta)
 
@@ -167,7 +172,7 @@ def test_fp32_factor_with_fp64_refinement(device, kind, n, cut, cplx):
     rng = np.random.default_rng(4)
     b = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
     bd = to_dev(b)
-    x, info = ft.refine(bd, tol=1e-10, max_iters=30)
+    x, info = ft.refine(bd, tol=1e-10, max_iters=REFINE_CAP)
     xg = x.cpu().numpy()
     eta = backward_error(ip, ind, v, xg, b)
     assert info["converged"] and eta.max() <= 1e-10, (info, eta)
@@ -246,7 +251,7 @@ def test_batched_2d_problems_one_forest(device, precision):
     ft.factor(torch.tensor(A.data, device="cuda:0"), precision=precision)
     N = A.shape[0]
     b = rng.standard_normal((N, 16))
-    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=20)
+    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=REFINE_CAP)
     xg = x.cpu().numpy()
     o = 0
     for m in mats:
@@ -327,7 +332,7 @@ def test_block_ldl_fp32_with_refinement(device, kind, n, cut, cplx):
                        front_type="block_ldl")
     rng = np.random.default_rng(13)
     b = rng.standard_normal((N, 4)) + (1j * rng.standard_normal((N, 4)) if cplx else 0)
-    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=30)
+    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=REFINE_CAP)
     eta = backward_error(ip, ind, v, x.cpu().numpy(), b)
     assert info["converged"] and eta.max() <= 1e-10, (info, eta)
 
@@ -401,7 +406,7 @@ def test_grf_diffusion_matches_oracle(device, n, cut):
     assert rel_diff(xg, f.solve(b)) <= 1e-11
     assert backward_error(ip, ind, v, xg, b).max() <= 1e-14
     sym32, ft32 = gpu_tree(ip, ind, v, dims, cut, precision="f32")
-    xr, info = ft32.refine(to_dev(b), tol=1e-10, max_iters=30)
+    xr, info = ft32.refine(to_dev(b), tol=1e-10, max_iters=REFINE_CAP)
     assert info["converged"]
     assert backward_error(ip, ind, v, xr.cpu().numpy(), b).max() <= 1e-10
 

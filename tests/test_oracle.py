@@ -283,3 +283,30 @@ def test_planted_sign_error_is_caught():
                            t.array(s, oracle.OR_LOWER)]).astype(int)
     cols = np.arange(nd["off"], nd["off"] + nd["size"])
     assert np.abs(L - Ld[np.ix_(rows, cols)]).max() > 1e-3
+
+
+@pytest.mark.parametrize("kind,n,cut,cplx", [("lap", 30, 128, False), ("pml", 16, 64, True)])
+def test_dropped_panel_update_exceeds_fp32_front_bound(kind, n, cut, cplx):
+    """The GPU FP32 factor is checked front by front against the oracle
+    within F32_FRONT_BOUND (tests/test_gpu_configs.py). A factor that leaves
+    out one panel's trailing update (planted in the oracle: the root's first
+    panel, ldl/local_front.hpp:65-66) must miss that bound by far, so the
+    bound would catch such a mistake."""
+    from test_gpu_configs import F32_FRONT_BOUND
+    ip, ind, v, dims = problem(kind, n)
+    t = oracle.Tree(*dims, ip, ind, cutoff=cut)
+    good = oracle.Factor(t, is_complex=cplx)
+    good.initialize(ip, ind, v)
+    good.ldl(nb=96)
+    bad = oracle.Factor(t, is_complex=cplx)
+    bad.initialize(ip, ind, v)
+    bad.plant(t.num - 1, 0)
+    try:
+        bad.ldl(nb=96)
+        s = t.num - 1
+        Lg, dg = good.front(s)
+        Lb, db = bad.front(s)
+        worst = max(np.abs(Lb - Lg).max() / np.abs(Lg).max(), np.abs(db - dg).max() / np.abs(dg).max())
+    except ZeroDivisionError:
+        worst = np.inf   # the mistake even produced a zero pivot
+    assert worst > 1e3 * F32_FRONT_BOUND, worst
