@@ -49,7 +49,7 @@ EXPORTS = [
     "cliqg_spmv",
     "cliqg_plan_refine", "cliqg_plan_adjoint", "cliqg_plan_get_front",
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
-    "cliqg_plan_set_graphs",
+    "cliqg_plan_set_graphs", "cliqg_plan_solve_noalloc",
 ]
 
 
@@ -88,6 +88,7 @@ def lib():
         "cliqg_plan_factor": (c_int, [vp, vp, c_int, c_int]),
         "cliqg_plan_solve": (c_int, [vp, vp, c_int, c_int]),
         "cliqg_plan_solve_nodal": (c_int, [vp, vp, c_int, c_int]),
+        "cliqg_plan_solve_noalloc": (c_int, [vp, vp, c_int, c_int]),
         "cliqg_plan_multiply": (c_int, [vp, c_double, vp, c_int, c_double, vp, c_int, c_int]),
         "cliqg_spmv": (c_int, [c_int, vp, vp, vp, c_int, c_double, vp, c_int, c_double, vp, c_int,
                                 c_int, vp]),

@@ -258,6 +258,13 @@ cliqg_status cliqg_plan_solve_nodal(cliqg_plan plan, void* d_x, int ldx, int k) 
     });
 }
 
+cliqg_status cliqg_plan_solve_noalloc(cliqg_plan plan, void* d_x, int ldx, int k) {
+    return guarded([&] {
+        require(k >= 0 && (d_x || k == 0), "bad right-hand side");
+        P(plan).solve_noalloc(d_x, ldx, k);
+    });
+}
+
 cliqg_status cliqg_plan_multiply(cliqg_plan plan, double alpha, const void* d_x, int ldx,
                                  double beta, void* d_y, int ldy, int k) {
     return guarded([&] {

@@ -1235,6 +1235,15 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((k_widen<T, V>), dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
 }
 
+__global__ void k_fill_int(int* __restrict__ p, int n, int value) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = value;
+}
+void launch_fill_int(int* p, int n, int value, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3(grid_for(n)), dim3(256), 0, s, p, n, value);
+}
+
 // ---------------------------------------------------------------------------
 // explicit instantiations
 // ---------------------------------------------------------------------------

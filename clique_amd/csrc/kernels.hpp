@@ -225,5 +225,7 @@ void launch_adjoint_dA(int N, const int64_t* rowoff, const int* cols, const V* l
                        hipStream_t s);
 template <class T, class V>
 void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s);
+// p[0..n) := value (a kernel, so it can be recorded in a graph)
+void launch_fill_int(int* p, int n, int value, hipStream_t s);
 
 }  // namespace cliqg

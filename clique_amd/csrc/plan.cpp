@@ -13,13 +13,44 @@
 // the same grouped launches; its descriptor arrays are built once.
 #include "plan.hpp"
 
+#include "../../include/cliqg.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 
 namespace cliqg {
+
+namespace {
+
+// Scratch memory of the allocation-free solve (cliqg_plan_solve_noalloc),
+// one copy per device, held until the process ends. Host code never uses
+// the array's name for its memory: scratch_address() looks up each device's
+// address once, with hipGetSymbolAddress, and every kernel argument, copy
+// and clear uses that address.
+constexpr size_t NOALLOC_SCRATCH_BYTES = CLIQG_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex scratchMutex;
+std::map<int, char*> scratchAddress;   // device -> its copy of the array
+
+char* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(scratchMutex);
+    auto it = scratchAddress.find(device);
+    if (it != scratchAddress.end()) return it->second;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw Error(5, std::string("hipGetSymbolAddress of the scratch array failed: ") +
+                           hipGetErrorString(e));
+    scratchAddress[device] = static_cast<char*>(p);
+    return static_cast<char*>(p);
+}
+
+}  // namespace
 
 void DevBuf::alloc(size_t n) {
     release();
@@ -40,10 +71,13 @@ void upload(DevBuf& b, const std::vector<X>& v) {
     if (!v.empty()) CQ_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
 }
 
-// Descriptor arrays of a schedule, staged on the host and uploaded at once.
+// Descriptor arrays of a schedule, staged on the host and uploaded at once
+// (or, with `base` set, copied by the caller to device memory at `base`
+// before every use: the allocation-free solve).
 struct Arena {
     std::vector<char> host;
     DevBuf dev;
+    const char* base = nullptr;
     template <class X>
     size_t add(const std::vector<X>& v) {
         const size_t off = (host.size() + 255) & ~size_t(255);
@@ -58,7 +92,7 @@ struct Arena {
         std::vector<char>().swap(host);
     }
     template <class X> const X* at(size_t off) const {
-        return reinterpret_cast<const X*>(static_cast<const char*>(dev.p) + off);
+        return reinterpret_cast<const X*>((base ? base : static_cast<const char*>(dev.p)) + off);
     }
 };
 
@@ -356,12 +390,22 @@ struct Numeric : NumericBase {
     struct SolveSched {
         Arena arena;
         std::vector<Step> fwd, bwd;
+        // Workspace the descriptors point into: buffers owned here for the
+        // allocating entry points, the library's static scratch array for
+        // the allocation-free solve.
         DevBuf wpool, xn, xn2, tmp;
-        DevBuf splitWs;   // K slices of the split update-row products
+        DevBuf splitWs;          // K slices of the split update-row products
+        T* wp = nullptr;         // update rows W_s of every front, (sum of u) x k
+        T* x1 = nullptr;         // nodal right-hand sides N x k: sweep input and result
+        T* x2 = nullptr;         // second nodal buffer of the LDL sweeps (x1 for block fronts)
+        int* status = nullptr;   // zero-pivot report of the diagonal solve
+        int64_t wsUsed = 0;      // split-K workspace entries the schedule uses
         Graph graph;   // forward, diagonal and backward sweeps
     };
     Graph fgraph[2];   // LDL and block-LDL factorizations
     std::map<int, std::unique_ptr<SolveSched>> solves;
+    // schedules of the allocation-free solve (descriptors into the scratch array)
+    std::map<int, std::unique_ptr<SolveSched>> noallocSolves;
     DevBuf rbuf, identity;
     int rbufCols = 0;
 
@@ -797,9 +841,20 @@ struct Numeric : NumericBase {
     // Backward, per level from the root: W := X(lowerStruct), X_s -= L21^T W,
     // then per panel from the last X_k := Linv_k^T X_k and
     // X(0:k) -= L(k:k+b, 0:k)^T X_k.
+    // per-level n x k workspace of the block solves (entries)
+    int64_t solve_tmp_entries(int k) const {
+        if (!block()) return 0;
+        int64_t mx = 0;
+        for (int h = 0; h < P.numLevels; ++h) {
+            int64_t a = 0;
+            for (int s : P.levels[h]) a += (int64_t)P.fronts[s].n * k;
+            mx = std::max(mx, a);
+        }
+        return mx;
+    }
+
     SolveSched& schedule(int k) {
         const bool blk = block();
-        const int SB = P.sb;
         const int key = 2 * k + (blk ? 1 : 0);
         auto it = solves.find(key);
         if (it != solves.end()) return *it->second;
@@ -811,30 +866,109 @@ struct Numeric : NumericBase {
         // input in xn and leaves X in xn2, the backward sweep the reverse
         if (!blk) S.xn2.alloc(std::max<int64_t>((int64_t)P.N * k, 1) * sizeof(T));
         S.splitWs.alloc(split_bound() * sizeof(T));
-        T* ws = S.splitWs.template as<T>();
+        S.tmp.alloc(std::max<int64_t>(solve_tmp_entries(k), 1) * sizeof(T));
+        S.wp = S.wpool.template as<T>();
+        S.x1 = S.xn.template as<T>();
+        S.x2 = blk ? S.x1 : S.xn2.template as<T>();
+        S.status = P.dStatus.as<int>();
+        build_solve(S, k, S.tmp.template as<T>(), S.splitWs.template as<T>());
+        S.arena.finish();
+        SolveSched& ref = S;
+        solves.emplace(key, std::move(sp));
+        return ref;
+    }
+
+    // Schedule of the allocation-free solve for k right-hand sides: every
+    // buffer inside the scratch array at `base` (cap bytes), laid out as
+    // [descriptor arena | status | x1 | x2 | W pool | block workspace |
+    // split-K workspace]. Built twice: once to learn the arena's size and
+    // the split workspace the launches need, once with the final addresses.
+    SolveSched& noalloc_schedule(int k, char* base, size_t cap) {
+        const bool blk = block();
+        const int key = 2 * k + (blk ? 1 : 0);
+        auto it = noallocSolves.find(key);
+        if (it != noallocSolves.end()) return *it->second;
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t nk = (size_t)P.N * k * sizeof(T);
+        auto place = [&](SolveSched& S, char* b, size_t arenaBytes, int64_t wsEntries,
+                         T** tp, T** ws) {
+            size_t off = al(arenaBytes);
+            S.status = reinterpret_cast<int*>(b + off);
+            off += 256;
+            S.x1 = reinterpret_cast<T*>(b + off);
+            off += al(nk);
+            S.x2 = S.x1;
+            if (!blk) {
+                S.x2 = reinterpret_cast<T*>(b + off);
+                off += al(nk);
+            }
+            S.wp = reinterpret_cast<T*>(b + off);
+            off += al((size_t)P.wRows * k * sizeof(T));
+            *tp = reinterpret_cast<T*>(b + off);
+            off += al((size_t)solve_tmp_entries(k) * sizeof(T));
+            *ws = reinterpret_cast<T*>(b + off);
+            off += (size_t)wsEntries * sizeof(T);
+            return off;
+        };
+        T *tp = nullptr, *ws = nullptr;
+        size_t arenaBytes = 0;
+        int64_t wsEntries = 0;
+        {
+            SolveSched probe;
+            place(probe, base, 0, 0, &tp, &ws);
+            build_solve(probe, k, tp, ws);
+            arenaBytes = probe.arena.host.size();
+            wsEntries = probe.wsUsed;
+        }
+        auto sp = std::make_unique<SolveSched>();
+        SolveSched& S = *sp;
+        const size_t need = place(S, base, arenaBytes, wsEntries, &tp, &ws);
+        if (need > cap)
+            throw Error(1, "the allocation-free solve needs " + std::to_string(need >> 20) +
+                               " MiB of scratch for this plan and " + std::to_string(k) +
+                               " right-hand sides; its scratch array holds " +
+                               std::to_string(cap >> 20) + " MiB");
+        build_solve(S, k, tp, ws);
+        if (S.arena.host.size() != arenaBytes || S.wsUsed > wsEntries)
+            throw Error(5, "allocation-free solve schedule changed between its two builds");
+        S.arena.base = base;
+        SolveSched& ref = S;
+        noallocSolves.emplace(key, std::move(sp));
+        return ref;
+    }
+
+    // Solve schedule for k right-hand sides into S's workspace (tp: block
+    // workspace, ws: split-K workspace). The front's own rows are the
+    // nodal right-hand sides X (ld N); W_s (u x k, ld u) holds its update rows.
+    // Forward, per level: clear W, extend-add child slot 0 then 1, then per
+    // SB-wide panel X_k := Linv_k X_k and X(k+b:n) -= L(k+b:n, k) X_k, then
+    // W -= L21 X_s in one product.
+    // Backward, per level from the root: W := X(lowerStruct), X_s -= L21^T W,
+    // then per panel from the last X_k := Linv_k^T X_k and
+    // X(0:k) -= L(k:k+b, 0:k)^T X_k.
+    void build_solve(SolveSched& S, int k, T* tp, T* ws) {
+        const bool blk = block();
+        const int SB = P.sb;
         // per-front n x k workspace of the block solves
         std::vector<int64_t> tmpOff(P.numFronts, 0);
-        {
-            int64_t mx = 0;
-            for (int h = 0; h < P.numLevels; ++h) {
-                int64_t a = 0;
-                for (int s : P.levels[h]) {
-                    tmpOff[s] = a;
-                    a += (int64_t)(blk ? P.fronts[s].n : 0) * k;
-                }
-                mx = std::max(mx, a);
+        for (int h = 0; h < P.numLevels; ++h) {
+            int64_t a = 0;
+            for (int s : P.levels[h]) {
+                tmpOff[s] = a;
+                a += (int64_t)(blk ? P.fronts[s].n : 0) * k;
             }
-            S.tmp.alloc(std::max<int64_t>(mx, 1) * sizeof(T));
         }
-        T* wp = S.wpool.template as<T>();
-        T* xn = S.xn.template as<T>();
-        T* tp = S.tmp.template as<T>();
+        T* wp = S.wp;
+        T* xn = S.x1;
+        T* xn2 = S.x2;
         auto Tm = [&](int s) { return tp + tmpOff[s]; };
         const int64_t N = P.N;
         auto Wp = [&](int s) { return wp + P.fronts[s].W * k; };
         auto Xs = [&](int s) { return xn + P.fronts[s].off; };
-        T* xn2 = blk ? xn : S.xn2.template as<T>();
         auto X2 = [&](int s) { return xn2 + P.fronts[s].off; };
+        auto split = [&](StepBuilder<T>& sbl, const std::vector<GemmDesc<T>>& g) {
+            S.wsUsed = std::max(S.wsUsed, split_gemm(&sbl, g, ws));
+        };
         const int cA = P.conjugate ? G_CONJA : 0;
         for (int pass = 0; pass < 2; ++pass) {
             const bool bwd = pass == 1;
@@ -904,7 +1038,7 @@ struct Numeric : NumericBase {
                                                     N, f.n, k, f.n, 0});
                         }
                         sb.copy(cs);
-                        split_gemm(&sb, g, ws);
+                        split(sb, g);
                     }
                     // SB-wide panels: X2_p := inv(L_pp) X_p (into the second
                     // buffer), X(below) -= L(below, p) X2_p
@@ -934,7 +1068,7 @@ struct Numeric : NumericBase {
                         g.push_back(GemmDesc<T>{L(s) + f.n, X2(s), Wp(s), nullptr, nullptr, f.m, N,
                                                 f.u, f.u, k, f.n, G_NEG});
                     }
-                    split_gemm(&sb, g, ws);
+                    split(sb, g);
                     sextend(true);
                 } else {
                     std::vector<SolveSeg> segs;
@@ -967,11 +1101,11 @@ struct Numeric : NumericBase {
                             xs.push_back(GemmDesc<T>{L(s), Tm(s), Xs(s), nullptr, nullptr, f.m, f.n,
                                                      N, f.n, k, f.n, G_ACCUM | G_NEG});
                         }
-                        split_gemm(&sb, y, ws);
-                        split_gemm(&sb, xs, ws);
+                        split(sb, y);
+                        split(sb, xs);
                         continue;
                     }
-                    split_gemm(&sb, g, ws);
+                    split(sb, g);
                     // SB-wide panels from the last: X_p := inv(L_pp)^T X2_p (^H if
                     // Hermitian; back into xn), X2(0:w0) -= L(p, 0:w0)^T X_p
                     const int maxw = (maxn + SB - 1) / SB;
@@ -997,10 +1131,6 @@ struct Numeric : NumericBase {
                 }
             }
         }
-        S.arena.finish();
-        SolveSched& ref = S;
-        solves.emplace(key, std::move(sp));
-        return ref;
     }
 
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
@@ -1016,7 +1146,8 @@ struct Numeric : NumericBase {
                 continue;
             }
             hipEvent_t e0 = nullptr;
-            if (P.profiling) {
+            const bool prof = P.profiling && !a.base;   // never inside the allocation-free solve
+            if (prof) {
                 e0 = P.next_event();
                 CQ_HIP(hipEventRecord(e0, st));
             }
@@ -1043,8 +1174,7 @@ struct Numeric : NumericBase {
                     break;
                 case S_GATHER:
                     launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                     S->wpool.template as<T>(), S->xn.template as<T>(), P.N, k,
-                                     P.dLow.as<int>(), st);
+                                     S->wp, S->x1, P.N, k, P.dLow.as<int>(), st);
                     break;
                 case S_REDUCE:
                     launch_reduce_slices<T>(a.at<ReduceDesc<T>>(s.desc), a.at<int>(s.start),
@@ -1056,20 +1186,21 @@ struct Numeric : NumericBase {
                     break;
                 case S_SEXTEND:
                     launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
-                                           s.nwork, S->wpool.template as<T>(),
-                                           S->xn.template as<T>(), P.N, P.dRel.as<int>(), k, st);
+                                           s.nwork, S->wp, S->x1, P.N, P.dRel.as<int>(), k, st);
                     break;
             }
-            if (P.profiling) {
+            if (prof) {
                 hipEvent_t e1 = P.next_event();
                 CQ_HIP(hipEventRecord(e1, st));
                 const int kind = (s.kind == S_GEMM && S) ? 7 : s.kind;
                 // factor GEMMs by instantiation: pivot-scaled updates (Schur,
                 // panel and outer-panel updates) apart from the TRSM-like rest
-                // (32x32-tile updates apart: a different kernel)
+                // (32x32- and 128x128-tile updates apart: different kernels)
                 const char* name = (s.kind == S_GEMM && !S)
                                        ? ((s.update || s.scaled)
-                                              ? (s.variant == TILE_32 ? "gemm_update_small" : "gemm_update")
+                                              ? (s.variant == TILE_32    ? "gemm_update_small"
+                                                 : s.variant == TILE_128 ? "gemm_update_large"
+                                                                         : "gemm_update")
                                               : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
                                                              : step_name(kind);
@@ -1152,24 +1283,47 @@ struct Numeric : NumericBase {
             }
             map = identity.as<int>();
         }
-        launch_pull<T, V>(x, ldx, map, S.xn.template as<T>(), P.N, k, st);
+        launch_pull<T, V>(x, ldx, map, S.x1, P.N, k, st);
         solve_core(S, k);
-        launch_push<T, V>(S.xn.template as<T>(), map, x, ldx, P.N, k, st);
+        launch_push<T, V>(S.x1, map, x, ldx, P.N, k, st);
         CQ_HIP(hipGetLastError());
     }
 
-    // forward, diagonal and backward sweeps on the nodal buffer S.xn
+    // The allocation-free solve: the same sweeps with every buffer in the
+    // scratch array (no device memory is allocated or freed; the first call
+    // for a right-hand-side count builds its launch schedule on the host).
+    void solve_noalloc(V* x, int64_t ldx, int k, char* base, size_t cap) {
+        if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
+        SolveSched& S = noalloc_schedule(k, base, cap);
+        hipStream_t st = P.stream;
+        // the scratch array holds the last call's data: this call's
+        // descriptors go to its head, its status word is cleared
+        CQ_HIP(hipMemcpyAsync(base, S.arena.host.data(), S.arena.host.size(),
+                              hipMemcpyHostToDevice, st));
+        launch_fill_int(S.status, 4, 0, st);
+        launch_pull<T, V>(x, ldx, P.dMap.as<int>(), S.x1, P.N, k, st);
+        solve_core(S, k);
+        launch_push<T, V>(S.x1, P.dMap.as<int>(), x, ldx, P.N, k, st);
+        CQ_HIP(hipGetLastError());
+        int h[3] = {0, 0, 0};
+        CQ_HIP(hipMemcpyAsync(h, S.status, sizeof(h), hipMemcpyDeviceToHost, st));
+        CQ_HIP(hipStreamSynchronize(st));
+        if (h[0] != 0)
+            throw Error(2, "Solve: zero or non-finite pivot at nodal index " + std::to_string(h[2]));
+    }
+
+    // forward, diagonal and backward sweeps on the nodal buffer S.x1 (graph
+    // replay only for the plan's own schedules)
     void solve_core(SolveSched& S, int k) {
         auto body = [&] {
             run(S.fwd, S.arena, &S, k);
             // block fronts hold inv(ATL), so there is no separate diagonal
             // solve (solve.hpp:48-53)
             if (!block())
-                launch_diag_solve<T, V>(S.xn2.template as<T>(), dpool.as<V>(), P.N, k,
-                                        P.dStatus.as<int>(), P.stream);
+                launch_diag_solve<T, V>(S.x2, dpool.as<V>(), P.N, k, S.status, P.stream);
             run(S.bwd, S.arena, &S, k);
         };
-        if (!replay(S.graph, body)) body();
+        if (S.arena.base || !replay(S.graph, body)) body();
     }
 
     V* scratch_cols(int k) {
@@ -1315,6 +1469,7 @@ Plan::Plan(std::shared_ptr<const Symbolic> symIn, const int64_t* rowOff, const i
         }
     }
 
+    CQ_HIP(hipGetDevice(&device));
     if (!stream) {
         CQ_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         ownStream = true;
@@ -1474,6 +1629,24 @@ void Plan::solve(void* d_x, int64_t ldx, int k, bool nodal) {
     check_status("Solve");
 }
 
+void Plan::solve_noalloc(void* d_x, int64_t ldx, int k) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (k <= 0) return;
+    if (ldx < N) throw Error(1, "leading dimension smaller than N");
+    int dev = -1;
+    CQ_HIP(hipGetDevice(&dev));
+    if (dev != device)
+        throw Error(1, "the allocation-free solve must run with the plan's device current");
+    char* base = scratch_address(dev);   // first call on this device: the lookup
+    after_caller();
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        static_cast<Numeric<T>*>(num.get())->solve_noalloc(static_cast<V*>(d_x), ldx, k, base,
+                                                           NOALLOC_SCRATCH_BYTES);
+    });
+}
+
 void Plan::multiply(double alpha, const void* d_x, int64_t ldx, double beta, void* d_y,
                     int64_t ldy, int k) {
     after_caller();
@@ -1509,7 +1682,7 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
         CQ_HIP(hipMemsetAsync(nrm + 1, 0, sizeof(unsigned long long) * 3 * k, stream));
         if (k > 65535) throw Error(1, "at most 65535 right-hand sides per solve");
         auto& S = nm->schedule(k);
-        T* xnb = S.xn.template as<T>();
+        T* xnb = S.x1;
         // x0 = solve(b): b goes straight into the solve's nodal buffer
         launch_col_abs_max<V>(b, ldb, N, k, nrm + 1, stream);
         launch_pull<T, V>(b, ldb, dMap.as<int>(), xnb, N, k, stream);

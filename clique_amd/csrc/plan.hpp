@@ -61,6 +61,9 @@ public:
 
     void factor(const void* d_vals, int precision, int frontType);
     void solve(void* d_x, int64_t ldx, int k, bool nodal);
+    // solve() without allocating device memory (all workspace in the
+    // library's static scratch array of the plan's device)
+    void solve_noalloc(void* d_x, int64_t ldx, int k);
     void multiply(double alpha, const void* d_x, int64_t ldx, double beta, void* d_y,
                   int64_t ldy, int k);
     void refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, double tol,
@@ -84,6 +87,7 @@ public:
     std::vector<int64_t> hostRowOff;
     hipStream_t stream = nullptr;
     bool ownStream = false;
+    int device = 0;   // the device current at creation (the plan's memory lives there)
     // side stream (default priority) for work that overlaps the main
     // sequence, and the events that order the two
     hipStream_t stream2 = nullptr;

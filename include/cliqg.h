@@ -145,6 +145,22 @@ cliqg_status cliqg_plan_solve(cliqg_plan plan, void* d_x, int ldx, int k);
 /* The same on a nodal-ordered right-hand side (no Pull/Push). */
 cliqg_status cliqg_plan_solve_nodal(cliqg_plan plan, void* d_x, int ldx, int k);
 
+/* cliqg_plan_solve without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and results as cliqg_plan_solve, but every workspace
+ * buffer (nodal right-hand sides, update rows, split-K slices, launch
+ * descriptors, status word) lives in the library's static device array of
+ * CLIQG_NOALLOC_SCRATCH_BYTES per device, held until the process ends; the
+ * call allocates and frees no device memory (the first call for a plan and
+ * right-hand-side count builds that schedule's descriptors on the host).
+ * A solve whose workspace exceeds the array is rejected with
+ * CLIQG_ERR_INVALID. All calls on one device share the array: calls that
+ * overlap in time on one device race (run them on one stream, or in
+ * sequence). The plan's device must be current. */
+#define CLIQG_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+cliqg_status cliqg_plan_solve_noalloc(cliqg_plan plan, void* d_x, int ldx, int k);
+
 /* y := alpha A x + beta y with the factored values, FP64 (replaces
  * cliq::Multiply, numeric/multiply.hpp:27-166, at world size 1). */
 cliqg_status cliqg_plan_multiply(cliqg_plan plan, double alpha, const void* d_x, int ldx,
