@@ -104,34 +104,60 @@ def symbolic_for(ip, ind, dims, cutoff=128):
     return cq.Symbolic.concat(syms)
 
 
-def cpu_baseline(budget_s=20.0):
-    """Oracle FP64 LDL (the reference's CPU algorithm restated in C, OpenMP)
-    on a bounded sample of the same operator family; TFLOP/s."""
-    import oracle
+def pml_sample(n):
+    """The config-5 operator scaled down to n^3: omega keeps config 5's points
+    per wavelength (omega h = 2 pi 10 / 121), the PML its share of the grid."""
     from clique_amd import problems as P
+    return P.helmholtz_pml(n, omega=2 * np.pi * 10 * (n + 1) / 121.0, pml=max(2, round(10 * n / 120)),
+                           strength=20.0)
+
+
+def cpu_baseline(budget_s=20.0, k=64, target=None):
+    """The reference's CPU path restated (oracle/oracle_blas.inc: sequential
+    postorder front loop, FrontLDL with nb = 96 making its Trsm / Gemm / Trrk
+    calls through a multithreaded CPU BLAS, BLAS-backed front solves), FP64
+    complex, on the largest sample of the config-5 operator family the budget
+    allows. LDL and Solve (k point sources) are timed separately, as
+    tests/Solve.cpp:223-302 does; target = (FactorizationWork, SolveWork per
+    RHS) of the benchmarked workload, whose CPU times are extrapolated by the
+    work ratio."""
+    import oracle
     threads = min(16, len(os.sched_getaffinity(0)))
-    oracle.build()
-    n = 24
-    best = None
+    blas = oracle.blas_load(threads)
+    rng = np.random.default_rng(7)
+    n, best = 28, None
     while True:
-        ip, ind, v, dims = P.helmholtz_pml(n, omega=2 * np.pi * 10, pml=max(2, n // 12))
+        ip, ind, v, dims = pml_sample(n)
         t = oracle.Tree(*dims, ip, ind, cutoff=128)
         f = oracle.Factor(t, is_complex=True)
         f.initialize(ip, ind, v)
         t0 = time.perf_counter()
-        f.ldl(nb=96, nthreads=threads)
-        dt = time.perf_counter() - t0
-        flops = t.work(True)[0]
-        best = (n, flops, dt)
-        # grow until one run takes a good fraction of the budget
-        nxt = int(n * 1.25)
-        if dt * (nxt / n) ** 6 > budget_s or n >= 120:
+        f.ldl_blas(nb=96)
+        dl = time.perf_counter() - t0
+        N = ip.size - 1
+        b = np.zeros((N, k), dtype=np.complex128)
+        b[rng.choice(N, size=k, replace=False), np.arange(k)] = 1.0
+        t0 = time.perf_counter()
+        f.solve_nodal_blas(b)
+        ds = time.perf_counter() - t0
+        fw, sw, _ = t.work(True)
+        best = (n, fw, sw, dl, ds)
+        del f, t
+        nxt = int(n * 1.2)
+        if (dl + ds) * (nxt / n) ** 6 > budget_s or n >= 120:
             break
         n = nxt
-    n, flops, dt = best
-    return {"value": flops / dt / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle FP64 complex LDL (FrontLDL nb=96, OpenMP) of the {n}^3 PML "
-                       f"Helmholtz operator: {flops / 1e9:.1f} GFLOP in {dt:.2f} s")}
+    n, fw, sw, dl, ds = best
+    out = {"value": fw / dl / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+           "blas": blas, "ldl_s": dl, "solve_s": ds, "solve_rhs": k,
+           "sample": (f"reference CPU path restated (FrontLDL nb=96 through {blas}), FP64 "
+                      f"complex, {n}^3 PML Helmholtz sample: LDL {fw / 1e9:.1f} GFLOP in "
+                      f"{dl:.2f} s, Solve of {k} point sources in {ds:.2f} s")}
+    if target:
+        ft, st = target
+        out["extrapolated_to_workload"] = {"ldl_s": ft / (fw / dl), "solve_s": ds * st / sw,
+                                           "time_to_solution_s": ft / (fw / dl) + ds * st / sw}
+    return out
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gemm_update_pml120_fp32_traffic.json")
@@ -175,6 +201,37 @@ def factor_only(args):
                       "gemm_update_tflops": g["flops"] / (g["ms"] * 1e-3) / 1e12}))
 
 
+def timed_solve(p, src, reps=3):
+    """Median of `reps` solves of the right-hand sides src (original
+    ordering, Pull and Push included) on the plan's stream, in ms."""
+    import torch
+    x = src.clone()
+    ts = []
+    for r in range(reps + 1):
+        x.copy_(src)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        p.solve(x)
+        e1.record()
+        torch.cuda.synchronize()
+        if r:
+            ts.append(e0.elapsed_time(e1))
+    return float(np.median(ts))
+
+
+def solve_work(sym, flops_per_rhs, N, k, cplx, fac_bytes, rhs_bytes, ms):
+    """Algorithmic work of one solve with k right-hand sides: flops =
+    2 SolveWork k + N k (x4 complex for the diagonal), bytes = 2 sum(m n)
+    factor entries + 4 N k + 2 sum(u) k right-hand-side entries."""
+    _, _, entries = sym.work(cplx)
+    sum_u = sum(sym.node(s)["lower_size"] for s in range(sym.num_nodes))
+    fl = 2.0 * flops_per_rhs * k + N * k * (4 if cplx else 1)
+    by = 2.0 * entries * fac_bytes + 4.0 * N * k * rhs_bytes + 2.0 * sum_u * k * rhs_bytes
+    return {"ms": ms, "rhs": k, "flops": fl, "bytes": by, "tflops": fl / (ms * 1e-3) / 1e12,
+            "gbps": by / (ms * 1e-3) / 1e9, "hbm_frac": by / (ms * 1e-3) / 1e9 / HBM_PEAK}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +248,16 @@ def main():
     ap.add_argument("--no-block", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--factor-only", action="store_true",
                     help="two FP32 factorizations and a calibration copy (for counter runs)")
+    ap.add_argument("--cpu-baseline-only", type=str, default=None, metavar="K,FLOPS,SOLVE",
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        # the CPU baseline runs in a process of its own, without torch (whose
+        # bundled MKL would stand in for the CPU BLAS's symbols)
+        kk, ft, st = args.cpu_baseline_only.split(",")
+        tgt = (float(ft), float(st)) if float(ft) > 0 else None
+        print(json.dumps(cpu_baseline(args.cpu_budget, k=int(kk), target=tgt)))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the multifrontal factor and solves run on "
               "one MI355X here, and runs on several GPUs are not implemented", file=sys.stderr)
@@ -246,6 +312,12 @@ def main():
     print(f"# fp32+IR: {t32 * 1e3:.1f} ms/step, factor {np.mean(f32_ms):.1f} ms, "
           f"IR iterations {[i['iterations'] for i in infos]}, eta {infos[-1]['backward_error']:.2e}",
           file=sys.stderr, flush=True)
+    solve32_ms = timed_solve(p32, bd)
+    fac_bytes = 8 if cplx else 4
+    sw32 = solve_work(sym, flops_per_rhs=solve_flops, N=N, k=k, cplx=cplx, fac_bytes=fac_bytes,
+                      rhs_bytes=fac_bytes, ms=solve32_ms)
+    print(f"# fp32 solve ({k} RHS): {solve32_ms:.2f} ms, {sw32['gbps']:.0f} GB/s, "
+          f"{sw32['tflops']:.1f} TFLOP/s", file=sys.stderr, flush=True)
 
     mem32 = p32.memory()
     del p32
@@ -311,10 +383,17 @@ def main():
             infos64.append(info)
         torch.cuda.synchronize()
         t64 = (time.perf_counter() - t0) / args.steps
+        solve64_ms = timed_solve(p64, bd)
+        sw64 = solve_work(sym, flops_per_rhs=solve_flops, N=N, k=k, cplx=cplx,
+                          fac_bytes=2 * fac_bytes, rhs_bytes=2 * fac_bytes, ms=solve64_ms)
+        # forward error of the FP32+IR solution against the FP64 one, per column
+        fwd = (torch.linalg.vector_norm(x32 - x64, dim=0) /
+               torch.linalg.vector_norm(x64, dim=0)).cpu().numpy()
         p64.set_profiling(True)
         p64.factor(vals, precision="f64")
         prof64 = p64.profile()
         fp64 = {"time_to_tol_ms": t64 * 1e3, "factor_ms": float(np.mean(f64_ms)),
+                "solve": sw64,
                 "factor_tflops": flops / (np.mean(f64_ms) * 1e-3) / 1e12,
                 "refine_iterations": infos64[-1]["iterations"],
                 "backward_error": infos64[-1]["backward_error"],
@@ -326,6 +405,9 @@ def main():
 
     g = prof32["gemm_update"]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
+    # complex FP32 products issue three real MFMAs per complex block (Gauss)
+    # where the flop convention counts four real multiplications
+    mfma_share = 0.75 if cplx else 1.0
     traffic, traffic_src = measured_traffic(args.config, args.n)
     solve_names = ("solve_gemm", "solve_extend_add", "solve_gather", "solve_zero", "solve_reduce")
 
@@ -356,13 +438,18 @@ def main():
         "fp32_ir": {"time_to_tol_ms": t32 * 1e3, "factor_ms": fac32,
                     "refine_iterations": infos[-1]["iterations"],
                     "backward_error": infos[-1]["backward_error"], "tol": args.tol,
-                    "device_bytes": mem32},
+                    "device_bytes": mem32, "solve": sw32,
+                    "forward_error_vs_fp64": ({"max": float(fwd.max()),
+                                               "median": float(np.median(fwd))}
+                                              if fp64 else None)},
         "fp64": fp64,
         "speedup_fp32ir_vs_fp64": (fp64["time_to_tol_ms"] / (t32 * 1e3)) if fp64 else None,
         "roofline": {"bound": "mfma",
                      "kernel": "k_gemm<cfloat,64,64,3> (Schur, panel and outer-panel updates, FP32 factor)",
                      "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["fp32"], "traffic": traffic,
+                     "frac": achieved / PEAK["fp32"],
+                     "mfma_issue_frac": achieved * mfma_share / PEAK["fp32"],
+                     "traffic": traffic,
                      "traffic_source": traffic_src,
                      "launches": g["launches"], "avg_launch_ms": g["ms"] / g["launches"],
                      "flops_per_launch": g["flops"] / g["launches"]},
@@ -371,7 +458,18 @@ def main():
         "block_ldl_fp32_ir": blk,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+        import subprocess
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-budget",
+                            str(args.cpu_budget), "--cpu-baseline-only",
+                            f"{k},{flops if cplx else 0},{solve_flops}"],
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("CPU baseline failed: " + r.stderr[-2000:])
+        cb = json.loads(r.stdout.strip().splitlines()[-1])
+        out["cpu_baseline"] = cb
+        if "extrapolated_to_workload" in cb:
+            out["speedup_fp32ir_vs_cpu_extrapolated"] = (
+                cb["extrapolated_to_workload"]["time_to_solution_s"] / t32)
     print(json.dumps(out))
 
 

@@ -50,6 +50,9 @@ def lib():
         L.or_ldl.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_solve.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int]
         L.or_plant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_blas_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+        L.or_ldl_blas.argtypes = [vp, ctypes.c_int]
+        L.or_solve_blas.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int]
         L.or_get_front.argtypes = [vp, ctypes.c_int, dp, dp]
         L.or_multiply.argtypes = [ctypes.c_int, lp, ip, dp, ctypes.c_int, ctypes.c_double,
                                   dp, ctypes.c_int, ctypes.c_double, dp, ctypes.c_int,
@@ -57,6 +60,42 @@ def lib():
         L.or_factor_work.argtypes = [vp, ctypes.c_int, dp, dp, dp]
         _lib = L
     return _lib
+
+
+_BLAS = None
+
+
+def _blas_candidates():
+    import glob
+    cands = [("/opt/conda/lib/libmkl_rt.so", "", "MKL (libmkl_rt)")]
+    try:
+        import scipy
+        root = os.path.dirname(os.path.dirname(scipy.__file__))
+        for p in sorted(glob.glob(os.path.join(root, "scipy.libs", "libscipy_openblas*.so"))):
+            cands.append((p, "scipy_", "OpenBLAS (scipy bundle)"))
+    except ImportError:
+        pass
+    return cands
+
+
+def blas_load(threads: int) -> str:
+    """Load a multithreaded CPU BLAS for the CPU-baseline path (MKL if
+    present, else scipy's bundled OpenBLAS); returns its name. In a process
+    that has imported torch, MKL is skipped: torch's own MKL and OpenMP
+    symbols take the place of a second MKL's and its results are wrong
+    (scipy's OpenBLAS has prefixed symbols and is unaffected)."""
+    import sys
+    global _BLAS
+    if _BLAS is not None:
+        return _BLAS
+    for path, prefix, name in _blas_candidates():
+        if prefix == "" and "torch" in sys.modules:
+            continue
+        if os.path.exists(path) and lib().or_blas_load(path.encode(), prefix.encode(),
+                                                      int(threads)) == 0:
+            _BLAS = f"{name}, {threads} threads"
+            return _BLAS
+    raise RuntimeError("no CPU BLAS could be loaded for the CPU baseline")
 
 
 def _ip(a):
@@ -92,8 +131,8 @@ class Tree:
         self.num = lib().or_tree_num_nodes(self._h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().or_tree_free(self._h)
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_tree_free(self._h)
             self._h = None
 
     def node(self, s):
@@ -131,8 +170,8 @@ class Factor:
         self._h = ctypes.c_void_p(h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().or_factor_free(self._h)
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.or_factor_free(self._h)
             self._h = None
 
     def initialize(self, indptr, indices, vals):
@@ -152,6 +191,17 @@ class Factor:
         if r != 0:
             raise ZeroDivisionError(f"oracle LDL hit a zero pivot ({r})")
 
+    def ldl_blas(self, nb=96):
+        """LocalLDL with FrontLDL's dense calls made through the CPU BLAS
+        loaded by blas_load() (the CPU baseline of the benchmark)."""
+        r = lib().or_ldl_blas(self._h, nb)
+        if r != 0:
+            raise RuntimeError(f"oracle BLAS LDL failed ({r})")
+
+    def solve_nodal_blas(self, x):
+        """solve_nodal() with the front solves made through the CPU BLAS."""
+        return self._solve_nodal(x, lib().or_solve_blas)
+
     def plant(self, front, panel):
         """Test hook: the next ldl() skips the trailing update of one panel
         of one front (a deliberate mistake the tolerances must catch)."""
@@ -159,6 +209,9 @@ class Factor:
 
     def solve_nodal(self, x):
         """Solve A_nodal y = x for x in nodal ordering (N or N x k)."""
+        return self._solve_nodal(x, lib().or_solve)
+
+    def _solve_nodal(self, x, fn):
         dt = np.complex128 if self.is_complex else np.float64
         X = np.array(x, dtype=dt, order="F", copy=True)
         squeeze = X.ndim == 1
@@ -166,7 +219,7 @@ class Factor:
             X = X.reshape(-1, 1, order="F")
         k = X.shape[1]
         buf = X.view(np.float64) if not self.is_complex else X.reshape(-1, order="F").view(np.float64)
-        r = lib().or_solve(self._h, _dp(buf), self.tree.N, k)
+        r = fn(self._h, _dp(buf), self.tree.N, k)
         if r != 0:
             raise RuntimeError(f"or_solve failed ({r})")
         if self.is_complex:

@@ -5,6 +5,8 @@
 #include "oracle.h"
 
 #include <complex.h>
+#include <dlfcn.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -338,21 +340,88 @@ struct or_factor {
     void** diag;   /* per front n     */
 };
 
+/* A multithreaded BLAS loaded at run time for the CPU-baseline path
+ * (Fortran interfaces; the trailing lengths are the hidden character-argument
+ * lengths of gfortran-built libraries, ignored by the others). */
+typedef void (*gemm_fn)(const char*, const char*, const int*, const int*, const int*, const void*,
+                        const void*, const int*, const void*, const int*, const void*, void*,
+                        const int*, size_t, size_t);
+typedef void (*trsm_fn)(const char*, const char*, const char*, const char*, const int*,
+                        const int*, const void*, const void*, const int*, void*, const int*,
+                        size_t, size_t, size_t, size_t);
+typedef void (*gemmt_fn)(const char*, const char*, const char*, const int*, const int*,
+                         const void*, const void*, const int*, const void*, const int*,
+                         const void*, void*, const int*, size_t, size_t, size_t);
+typedef struct {
+    gemm_fn dgemm, zgemm;
+    trsm_fn dtrsm, ztrsm;
+    gemmt_fn dgemmt, zgemmt;
+    int loaded;
+} or_blas;
+static or_blas g_blas;
+
 #define T double
 #define SUF d
+#define BLP d
 #define CONJ(x) (x)
 #include "oracle_num.inc"
+#include "oracle_blas.inc"
 #undef T
 #undef SUF
+#undef BLP
 #undef CONJ
 
 #define T double complex
 #define SUF z
+#define BLP z
 #define CONJ(x) (f->conjugate ? conj(x) : (x))
 #include "oracle_num.inc"
+#include "oracle_blas.inc"
 #undef T
 #undef SUF
+#undef BLP
 #undef CONJ
+
+int or_blas_load(const char* path, const char* prefix, int threads) {
+    void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    char name[128];
+#define OR_SYM(field, type, base)                                   \
+    snprintf(name, sizeof(name), "%s" base, prefix);                \
+    g_blas.field = (type)dlsym(h, name);                            \
+    if (!g_blas.field) return -2;
+    OR_SYM(dgemm, gemm_fn, "dgemm_")
+    OR_SYM(zgemm, gemm_fn, "zgemm_")
+    OR_SYM(dtrsm, trsm_fn, "dtrsm_")
+    OR_SYM(ztrsm, trsm_fn, "ztrsm_")
+    OR_SYM(dgemmt, gemmt_fn, "dgemmt_")
+    OR_SYM(zgemmt, gemmt_fn, "zgemmt_")
+#undef OR_SYM
+    /* MKL keeps its default (Intel OpenMP) threading layer: its GNU layer
+     * ignored the thread count here */
+    void (*mkl_threads)(int) = (void (*)(int))dlsym(h, "MKL_Set_Num_Threads");
+    snprintf(name, sizeof(name), "%sopenblas_set_num_threads", prefix);
+    void (*ob_threads)(int) = (void (*)(int))dlsym(h, name);
+    if (threads > 0) {
+        if (mkl_threads) mkl_threads(threads);
+        if (ob_threads) ob_threads(threads);
+    }
+    g_blas.loaded = 1;
+    return 0;
+}
+
+int or_ldl_blas(or_factor* f, int nb) {
+    if (!g_blas.loaded || nb < 1) return -1;
+    f->block = 0;
+    int r = f->is_complex ? ldl_blas_z(f, nb) : ldl_blas_d(f, nb);
+    if (r == 0) f->factored = 1;
+    return r;
+}
+
+int or_solve_blas(or_factor* f, double* x, int ldx, int k) {
+    if (!g_blas.loaded || !f->factored || f->block) return -1;
+    return f->is_complex ? solve_blas_z(f, x, ldx, k) : solve_blas_d(f, x, ldx, k);
+}
 
 or_factor* or_factor_create(const or_tree* t, int is_complex, int conjugate) {
     or_factor* f = (or_factor*)calloc(1, sizeof(or_factor));

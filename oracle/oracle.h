@@ -19,6 +19,9 @@
  *                      lower_solve/local_front_block.hpp:28-72,
  *                      lower_solve/dist.hpp:374-387, diagonal_solve.hpp:48-68
  *   or_multiply        numeric/multiply.hpp:149-165
+ *   or_ldl_blas,       the same LocalLDL and solves with FrontLDL's Trsm, Gemm
+ *   or_solve_blas      and Trrk and the front solves' Trsm and Gemm made
+ *                      through a CPU BLAS (the CPU baseline of bench.py)
  *   or_factor_work     numeric/dist_symm_front_tree/impl.hpp:555-585, 607-635
  *
  * Pinning (no reference golden vectors exist; Elemental, which holds the
@@ -72,6 +75,13 @@ void or_plant(or_factor* f, int front, int panel);
 int or_solve(or_factor* f, double* x, int ldx, int k);
 /* copy front s: L is m x n column-major (ld = m), diag n entries */
 int or_get_front(const or_factor* f, int s, double* L, double* diag);
+/* CPU baseline: the same LocalLDL / solves with the dense calls of
+ * FrontLDL and the front solves made through a multithreaded BLAS loaded from
+ * `path` (symbol names prefixed by `prefix`, e.g. "" for MKL, "scipy_" for
+ * scipy's OpenBLAS) with `threads` threads. 0 on success. */
+int or_blas_load(const char* path, const char* prefix, int threads);
+int or_ldl_blas(or_factor* f, int nb);
+int or_solve_blas(or_factor* f, double* x, int ldx, int k);
 /* y := alpha A x + beta y (real alpha, beta) in the original ordering */
 void or_multiply(int N, const long* rowoff, const int* cols, const double* vals,
                  int is_complex, double alpha, const double* x, int ldx,

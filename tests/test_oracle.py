@@ -310,3 +310,33 @@ def test_dropped_panel_update_exceeds_fp32_front_bound(kind, n, cut, cplx):
     except ZeroDivisionError:
         worst = np.inf   # the mistake even produced a zero pivot
     assert worst > 1e3 * F32_FRONT_BOUND, worst
+
+
+@pytest.mark.parametrize("kind,n,cut,cplx,conj", [("lap", 12, 32, False, False),
+                                                  ("pml", 12, 32, True, False),
+                                                  ("herm", 6, 16, True, True)])
+def test_blas_baseline_matches_oracle(kind, n, cut, cplx, conj):
+    """The CPU-baseline path (the same LocalLDL and solves with the dense
+    calls made through a CPU BLAS) reproduces the pinned oracle's fronts and
+    solutions to rounding."""
+    from helpers import hermitian_matrix
+    oracle.blas_load(4)
+    ip, ind, v, dims = hermitian_matrix(n) if kind == "herm" else problem(kind, n)
+    t = oracle.Tree(*dims, ip, ind, cutoff=cut)
+    ref = oracle.Factor(t, is_complex=cplx, conjugate=conj)
+    ref.initialize(ip, ind, v)
+    ref.ldl(nb=8)
+    f = oracle.Factor(t, is_complex=cplx, conjugate=conj)
+    f.initialize(ip, ind, v)
+    f.ldl_blas(nb=8)
+    for s in range(t.num):
+        La, da = ref.front(s)
+        Lb, db = f.front(s)
+        assert np.abs(La - Lb).max(initial=0) <= 1e-12 * max(np.abs(La).max(initial=0), 1)
+        assert np.abs(da - db).max(initial=0) <= 1e-12 * max(np.abs(da).max(initial=0), 1)
+    N = ip.size - 1
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    xa = ref.solve_nodal(b)
+    xb = f.solve_nodal_blas(b)
+    assert rel_diff(xb, xa) <= 1e-12
