@@ -257,6 +257,19 @@ class DistSymmFrontTree:
             return L, d[:0]
         return L, d[:n]
 
+    def set_front(self, s, L):
+        """Stage front s's unfactored values (an m x n array, FP64 field type)
+        for the next factor(), which uses them in place of A's assembled
+        values for that front (tests/Refactor.cpp:150-160 writes
+        localFronts[s].frontL and factors again)."""
+        nd = self.sym.node(s)
+        n, m = nd["size"], nd["size"] + nd["lower_size"]
+        dt = np.complex128 if self.is_complex else np.float64
+        A = np.asfortranarray(np.asarray(L, dtype=dt))
+        if A.shape != (m, n):
+            raise ValueError(f"front {s} is {m} x {n}, got {A.shape}")
+        _lib.check(lib().cliqg_plan_set_front(self._h, s, A.ctypes.data_as(ctypes.c_void_p)))
+
     def memory(self):
         b = ctypes.c_size_t()
         _lib.check(lib().cliqg_plan_memory(self._h, ctypes.byref(b)))

@@ -294,6 +294,55 @@ cliqg_status cliqg_spmv(int N, const int64_t* d_row_off, const int* d_cols, cons
     });
 }
 
+cliqg_status cliqg_permute_rows(int N, int k, const int* d_perm, int scatter, const void* d_src,
+                                int lds, void* d_dst, int ldd, cliqg_field field,
+                                void* hip_stream) {
+    return guarded([&] {
+        require(N >= 0 && k >= 0 && lds >= N && ldd >= N, "bad dimensions");
+        if (N == 0 || k == 0) return;
+        require(d_perm && d_src && d_dst, "null argument");
+        require(field == CLIQG_REAL || field == CLIQG_COMPLEX, "unknown field");
+        const size_t vb = field == CLIQG_COMPLEX ? 16 : 8;
+        const char* s0 = static_cast<const char*>(d_src);
+        const char* d0 = static_cast<const char*>(d_dst);
+        const char* s1 = s0 + ((size_t)lds * (k - 1) + N) * vb;
+        const char* d1 = d0 + ((size_t)ldd * (k - 1) + N) * vb;
+        require(s1 <= d0 || d1 <= s0, "source and destination overlap");
+        hipStream_t st = static_cast<hipStream_t>(hip_stream);
+        if (field == CLIQG_COMPLEX)
+            cliqg::launch_permute_rows<cliqg::cdouble>((const cliqg::cdouble*)d_src, lds,
+                                                       (cliqg::cdouble*)d_dst, ldd, d_perm, N, k,
+                                                       scatter != 0, st);
+        else
+            cliqg::launch_permute_rows<double>((const double*)d_src, lds, (double*)d_dst, ldd,
+                                               d_perm, N, k, scatter != 0, st);
+        CQ_HIP(hipGetLastError());
+    });
+}
+
+cliqg_status cliqg_col_norms(int N, int k, const void* d_x, int ldx, cliqg_field field,
+                             double* h_norms, void* hip_stream) {
+    return guarded([&] {
+        require(N >= 0 && k >= 0 && ldx >= N, "bad dimensions");
+        if (k == 0) return;
+        require(d_x || N == 0, "null argument");
+        require(h_norms, "null output");
+        require(field == CLIQG_REAL || field == CLIQG_COMPLEX, "unknown field");
+        hipStream_t st = static_cast<hipStream_t>(hip_stream);
+        cliqg::DevBuf out((size_t)k * sizeof(double));
+        if (field == CLIQG_COMPLEX)
+            cliqg::launch_col_sumsq<cliqg::cdouble>((const cliqg::cdouble*)d_x, ldx, N, k,
+                                                    out.as<double>(), st);
+        else
+            cliqg::launch_col_sumsq<double>((const double*)d_x, ldx, N, k, out.as<double>(), st);
+        CQ_HIP(hipGetLastError());
+        CQ_HIP(hipMemcpyAsync(h_norms, out.p, (size_t)k * sizeof(double), hipMemcpyDeviceToHost,
+                              st));
+        CQ_HIP(hipStreamSynchronize(st));
+        for (int j = 0; j < k; ++j) h_norms[j] = std::sqrt(h_norms[j]);
+    });
+}
+
 cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* d_x, int ldx,
                                int k, double tol, int max_iters, cliqg_refine_info* info) {
     return guarded([&] {
@@ -330,6 +379,10 @@ cliqg_status cliqg_plan_adjoint(cliqg_plan plan, const void* d_g, int ldg, const
         p.adjoint(d_g, ldg, d_x, ldx, k, d_lambda, ldl, d_dA);
         CQ_HIP(hipStreamSynchronize(p.stream));
     });
+}
+
+cliqg_status cliqg_plan_set_front(cliqg_plan plan, int s, const void* h_L) {
+    return guarded([&] { P(plan).set_front(s, h_L); });
 }
 
 cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_diag) {

@@ -91,6 +91,8 @@ CQ_HD cdouble conj_(cdouble a) { return {a.x, -a.y}; }
 
 template <class T> CQ_HD T cj(T a, bool c) { return c ? conj_(a) : a; }
 
+CQ_HD double sq_mod(double a) { return a * a; }
+CQ_HD double sq_mod(cdouble a) { return a.x * a.x + a.y * a.y; }
 CQ_HD double abs1(double a) { return fabs(a); }
 CQ_HD double abs1(float a) { return fabs((double)a); }
 CQ_HD double abs1(cdouble a) { return hypot(a.x, a.y); }

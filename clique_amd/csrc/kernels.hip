@@ -1235,6 +1235,45 @@ void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((k_widen<T, V>), dim3(grid_for(n)), dim3(256), 0, s, src, dst, n);
 }
 
+template <class V>
+__global__ void k_permute_rows(const V* __restrict__ src, int64_t lds, V* __restrict__ dst,
+                               int64_t ldd, const int* __restrict__ perm, int N, bool scatter) {
+    const V* sc = src + (int64_t)blockIdx.y * lds;
+    V* dc = dst + (int64_t)blockIdx.y * ldd;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        if (scatter) dc[perm[p]] = sc[p];
+        else dc[p] = sc[perm[p]];
+    }
+}
+template <class V>
+void launch_permute_rows(const V* src, int64_t lds, V* dst, int64_t ldd, const int* perm, int N,
+                         int k, bool scatter, hipStream_t s) {
+    if (N <= 0 || k <= 0) return;
+    hipLaunchKernelGGL(k_permute_rows<V>, grid_cols(N, k), dim3(256), 0, s, src, lds, dst, ldd,
+                       perm, N, scatter);
+}
+
+template <class V>
+__global__ __launch_bounds__(256) void k_col_sumsq(const V* __restrict__ x, int64_t ldx, int N,
+                                                   double* out) {
+    __shared__ double red[256];
+    const V* xc = x + (int64_t)blockIdx.x * ldx;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < N; i += 256) acc += sq_mod(xc[i]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+template <class V>
+void launch_col_sumsq(const V* x, int64_t ldx, int N, int k, double* out, hipStream_t s) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_col_sumsq<V>, dim3(k), dim3(256), 0, s, x, ldx, N, out);
+}
+
 __global__ void k_fill_int(int* __restrict__ p, int n, int value) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = value;
@@ -1293,6 +1332,9 @@ CQ_INST_TV(cdouble, cdouble)
     template void launch_col_abs_max<V>(const V*, int64_t, int, int, unsigned long long*,       \
                                         hipStream_t);                                          \
     template void launch_copy_cols<V>(V*, int64_t, const V*, int64_t, int, int, hipStream_t);   \
+    template void launch_permute_rows<V>(const V*, int64_t, V*, int64_t, const int*, int, int,  \
+                                         bool, hipStream_t);                                    \
+    template void launch_col_sumsq<V>(const V*, int64_t, int, int, double*, hipStream_t);      \
     template void launch_adjoint_dA<V>(int, const int64_t*, const int*, const V*, int64_t,      \
                                        const V*, int64_t, int, bool, V*, hipStream_t);
 

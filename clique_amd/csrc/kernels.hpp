@@ -225,6 +225,14 @@ void launch_adjoint_dA(int N, const int64_t* rowoff, const int* cols, const V* l
                        hipStream_t s);
 template <class T, class V>
 void launch_widen(const T* src, V* dst, int64_t n, hipStream_t s);
+// Row permutation of an N x k block: gather dst(p, j) = src(perm[p], j), or
+// (scatter) dst(perm[p], j) = src(p, j)
+template <class V>
+void launch_permute_rows(const V* src, int64_t lds, V* dst, int64_t ldd, const int* perm, int N,
+                         int k, bool scatter, hipStream_t s);
+// out[j] = sum_i |x(i, j)|^2, one workgroup per column (summation order fixed)
+template <class V>
+void launch_col_sumsq(const V* x, int64_t ldx, int N, int k, double* out, hipStream_t s);
 // p[0..n) := value (a kernel, so it can be recorded in a graph)
 void launch_fill_int(int* p, int n, int value, hipStream_t s);
 

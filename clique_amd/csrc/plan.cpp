@@ -1258,15 +1258,30 @@ struct Numeric : NumericBase {
             blockBuilt = true;
         }
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
+        // host-staged fronts replace their assembled values: converted to
+        // the working type on the host, copied in after the assembly
+        std::vector<std::vector<T>> staged;
         auto body = [&] {
             launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate,
                                   st);
+            for (const auto& kv : P.stagedFronts) {
+                const FrontLayout& f = P.fronts[kv.first];
+                const size_t mn = (size_t)f.m * f.n;
+                const V* src = reinterpret_cast<const V*>(kv.second.data());
+                staged.emplace_back(mn);
+                for (size_t e = 0; e < mn; ++e) staged.back()[e] = cvt<T>(src[e]);
+                CQ_HIP(hipMemcpyAsync(L(kv.first), staged.back().data(), mn * sizeof(T),
+                                      hipMemcpyHostToDevice, st));
+            }
             if (block())
                 run(bsteps, barena, nullptr, 0);
             else
                 run(fsteps, farena, nullptr, 0);
         };
-        if (!replay(fgraph[block() ? 1 : 0], body)) body();
+        if (!P.stagedFronts.empty() || !replay(fgraph[block() ? 1 : 0], body)) body();
+        // the copies read the host buffers until they complete
+        CQ_HIP(hipStreamSynchronize(st));
+        P.stagedFronts.clear();
     }
 
     // x: V, N x k (ld ldx); nodal or original ordering; solved in place
@@ -1755,6 +1770,21 @@ void Plan::adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, i
                                  static_cast<V*>(d_dA), stream);
     });
     check_status("adjoint");
+}
+
+void Plan::set_front(int s, const void* hL) {
+    if (s < 0 || s >= numFronts) throw Error(1, "front index out of range");
+    const size_t mn = (size_t)fronts[s].m * fronts[s].n;
+    if (mn == 0) return;
+    if (!hL) throw Error(1, "null front values");
+    const size_t vb = complex ? 16 : 8;
+    std::vector<char>& buf = stagedFronts[s];
+    buf.resize(mn * vb);
+    std::memcpy(buf.data(), hL, mn * vb);
+    // only the lower trapezoid is a front's data: its strict upper triangle
+    // is kept zero, as the assembly leaves it
+    const int m = fronts[s].m, n = fronts[s].n;
+    for (int j = 1; j < n; ++j) std::memset(buf.data() + (size_t)j * m * vb, 0, (size_t)j * vb);
 }
 
 void Plan::get_front(int s, void* hL, void* hdiag) {

@@ -71,6 +71,9 @@ public:
     void adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, int k,
                  void* d_lambda, int64_t ldl, void* d_dA);
     void get_front(int s, void* h_L, void* h_diag);
+    // stage front s's unfactored values (m x n, FP64 field type) for the
+    // next factorization, which uses them in place of A's assembled values
+    void set_front(int s, const void* h_L);
     size_t device_bytes() const;
     double last_factor_ms() const { return lastFactorMs_; }
 
@@ -101,6 +104,7 @@ public:
     double normA = 0.0;
 
     std::unique_ptr<NumericBase> num;
+    std::map<int, std::vector<char>> stagedFronts;   // front -> m x n FP64 field values
     int check_status(const char* what);
 
     // opt-in per-launch profiling (HIP events around every scheduled launch)

@@ -173,6 +173,23 @@ cliqg_status cliqg_spmv(int N, const int64_t* d_row_off, const int* d_cols, cons
                         cliqg_field field, double alpha, const void* d_x, int ldx, double beta,
                         void* d_y, int ldy, int k, void* hip_stream);
 
+/* Row permutation of an N x k device block (FP64 or complex FP64), the
+ * device side of DistNodalMultiVec::Pull / Push at world size 1
+ * (dist_nodal_multi_vec/impl.hpp:67-217, 219-356): with scatter = 0,
+ * dst(p, j) = src(perm[p], j) (Pull with the inverse map); with scatter = 1,
+ * dst(perm[p], j) = src(p, j) (Push). d_perm holds N device ints; src and dst
+ * must not overlap. Asynchronous on hip_stream (NULL: the null stream). */
+cliqg_status cliqg_permute_rows(int N, int k, const int* d_perm, int scatter, const void* d_src,
+                                int lds, void* d_dst, int ldd, cliqg_field field,
+                                void* hip_stream);
+
+/* Two-norms of the k columns of an N x k device block (FP64 or complex
+ * FP64) into h_norms (host, k doubles), summed on the device in a fixed
+ * order (the role of DistMultiVec Norms, core/dist_multi_vec/impl.hpp).
+ * Synchronizes hip_stream. */
+cliqg_status cliqg_col_norms(int N, int k, const void* d_x, int ldx, cliqg_field field,
+                             double* h_norms, void* hip_stream);
+
 /* FP64 iterative refinement (no reference counterpart): x := solve(b); while
  * some column has backward error > tol and iterations < max_iters:
  * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified;
@@ -194,6 +211,15 @@ cliqg_status cliqg_plan_adjoint(cliqg_plan plan, const void* d_g, int ldg, const
  * precision), h_diag the n pivots (LDL fronts only; may be NULL). Replaces
  * reading frontTree.localFronts[s].frontL/diag (tests/Refactor.cpp:157-160). */
 cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_diag);
+
+/* Stage front s's unfactored values for the next cliqg_plan_factor, which
+ * factors them in place of A's assembled values for that front (the other
+ * fronts are assembled from A as usual); the staging is cleared by the
+ * factorization. h_L is the m x n front (column-major, ld m, FP64 or complex
+ * FP64); its lower trapezoid is read (rows >= column inside the n x n top
+ * block, all rows below it). Replaces writing
+ * frontTree.localFronts[s].frontL before LDL (tests/Refactor.cpp:150-160). */
+cliqg_status cliqg_plan_set_front(cliqg_plan plan, int s, const void* h_L);
 
 /* Timing of the last factorization on the plan's stream (HIP events). */
 cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms);

@@ -274,6 +274,7 @@ public:
         graph_.ResizeTo(N);
         vals_.clear();
         pending_.clear();
+        ++version_;
     }
     void StartAssembly() {
         if (assembling_) throw LogicError("Already assembling");
@@ -321,17 +322,31 @@ public:
     const cliq::DistGraph& LockedDistGraph() const { return graph_; }
     const std::vector<T>& Values() const { return vals_; }
     unsigned Version() const { return version_; }
+    // CSR arrays in device memory, uploaded once per assembly or value change
+    const int64_t* DeviceRowOffsets() const { sync(); return (const int64_t*)dOff_.get(); }
+    const int* DeviceCols() const { sync(); return (const int*)dCols_.get(); }
+    const void* DeviceValues() const { sync(); return dVals_.get(); }
 
 private:
     struct Entry {
         int i, j;
         T v;
     };
+    void sync() const {
+        if (devVersion_ == version_ + 1) return;
+        const cliq::DistGraph& g = graph_;
+        dOff_.upload(g.Offsets().data(), g.Offsets().size() * sizeof(int64_t));
+        dCols_.upload(g.Targets().data(), g.Targets().size() * sizeof(int));
+        dVals_.upload(vals_.data(), vals_.size() * sizeof(T));
+        devVersion_ = version_ + 1;
+    }
     cliq::DistGraph graph_;
     std::vector<T> vals_;
     std::vector<Entry> pending_;
     bool assembling_ = false;
     unsigned version_ = 0;
+    mutable unsigned devVersion_ = 0;   // version_ + 1 when the device copy is current
+    mutable detail::DeviceBuffer dOff_, dCols_, dVals_;
 };
 
 // ---------------------------------------------------------------------------
@@ -344,26 +359,73 @@ public:
     DistMultiVec(int height, int width, mpi::Comm comm = mpi::COMM_WORLD) : comm_(comm) {
         ResizeTo(height, width);
     }
+    DistMultiVec(const DistMultiVec& o) : comm_(o.comm_), h_(o.h_), w_(o.w_) {
+        o.host();
+        a_ = o.a_;
+        where_ = HOST;
+    }
+    DistMultiVec& operator=(const DistMultiVec& o) {
+        if (this != &o) {
+            o.host();
+            comm_ = o.comm_;
+            h_ = o.h_;
+            w_ = o.w_;
+            a_ = o.a_;
+            where_ = HOST;
+        }
+        return *this;
+    }
     void ResizeTo(int height, int width) {
         h_ = height;
         w_ = width;
         a_.assign((size_t)height * width, T(0));
+        where_ = HOST;
     }
     int Height() const { return h_; }
     int Width() const { return w_; }
     int LocalHeight() const { return h_; }
     int FirstLocalRow() const { return 0; }
+    int LDim() const { return std::max(h_, 1); }
     mpi::Comm Comm() const { return comm_; }
-    T GetLocal(int i, int j) const { return a_[i + (size_t)j * h_]; }
-    void SetLocal(int i, int j, T v) { a_[i + (size_t)j * h_] = v; }
-    void UpdateLocal(int i, int j, T v) { a_[i + (size_t)j * h_] += v; }
-    T* Buffer() { return a_.data(); }
-    const T* LockedBuffer() const { return a_.data(); }
+    // Host access: the values come back from the device copy when that is
+    // the newer one (after a device-side Push, Multiply or Refine).
+    T GetLocal(int i, int j) const { host(); return a_[i + (size_t)j * h_]; }
+    void SetLocal(int i, int j, T v) { hostMut(); a_[i + (size_t)j * h_] = v; }
+    void UpdateLocal(int i, int j, T v) { hostMut(); a_[i + (size_t)j * h_] += v; }
+    T* Buffer() { hostMut(); return a_.data(); }
+    const T* LockedBuffer() const { host(); return a_.data(); }
+    // Device access (HBM, column-major, ld LDim()): the host values go up
+    // once when they are the newer ones; DeviceBuffer() marks the device
+    // copy as the newer one.
+    const void* LockedDeviceBuffer() const { device(); return d_.get(); }
+    void* DeviceBuffer() { device(); where_ = DEVICE; return d_.get(); }
+    bool DeviceIsCurrent() const { return where_ != HOST; }
 
 private:
+    enum Where { BOTH, HOST, DEVICE };   // which copy holds the newest values
+    size_t bytes() const { return (size_t)h_ * w_ * sizeof(T); }
+    void host() const {
+        if (where_ == DEVICE) {
+            d_.download(a_.data(), bytes());
+            where_ = BOTH;
+        }
+    }
+    void hostMut() { host(); where_ = HOST; }
+    void device() const {
+        if (d_.size() != bytes()) {
+            d_.resize(bytes());
+            if (where_ != HOST && bytes()) where_ = HOST;   // fresh device memory
+        }
+        if (where_ == HOST) {
+            if (bytes()) d_.upload(a_.data(), bytes());
+            where_ = BOTH;
+        }
+    }
     mpi::Comm comm_;
     int h_ = 0, w_ = 0;
-    std::vector<T> a_;
+    mutable std::vector<T> a_;
+    mutable detail::DeviceBuffer d_;
+    mutable Where where_ = HOST;
 };
 
 template <typename T> void Zero(DistMultiVec<T>& X) { X.ResizeTo(X.Height(), X.Width()); }
@@ -391,9 +453,18 @@ template <typename T> void MakeUniform(DistMultiVec<T>& X, double radius = 1.0) 
         for (int i = 0; i < X.Height(); ++i) X.SetLocal(i, j, detail::sample_ball<T>(radius));
 }
 
+// column two-norms; on the device when the device copy is current
 template <typename T>
 void Norms(const DistMultiVec<T>& X, std::vector<typename Base<T>::type>& norms) {
     norms.assign(X.Width(), 0);
+    if (X.DeviceIsCurrent() && X.Height() > 0) {
+        std::vector<double> d(X.Width());
+        detail::check(cliqg_col_norms(X.Height(), X.Width(), X.LockedDeviceBuffer(), X.LDim(),
+                                      detail::Field<T>::value, d.data(), nullptr),
+                      "Norms");
+        for (int j = 0; j < X.Width(); ++j) norms[j] = d[j];
+        return;
+    }
     for (int j = 0; j < X.Width(); ++j) {
         double s = 0;
         for (int i = 0; i < X.Height(); ++i) s += std::norm(X.GetLocal(i, j));
@@ -421,11 +492,19 @@ public:
     int FirstLocalSource() const { return 0; }
     int Blocksize() const { return (int)map_.size(); }
     mpi::Comm Comm() const { return comm_; }
-    void Resize(int n) { map_.assign(n, 0); }
+    void Resize(int n) { map_.assign(n, 0); stale_ = true; }
     int GetLocal(int i) const { return map_[i]; }
-    void SetLocal(int i, int v) { map_[i] = v; }
-    std::vector<int>& Map() { return map_; }
+    void SetLocal(int i, int v) { map_[i] = v; stale_ = true; }
+    std::vector<int>& Map() { stale_ = true; return map_; }
     const std::vector<int>& Map() const { return map_; }
+    // the map in device memory (uploaded again after any change)
+    const int* DeviceMap() const {
+        if (stale_ || dev_.size() != map_.size() * sizeof(int)) {
+            dev_.upload(map_.data(), map_.size() * sizeof(int));
+            stale_ = false;
+        }
+        return static_cast<const int*>(dev_.get());
+    }
     void Translate(std::vector<int>& inds) const {
         for (int& i : inds) i = map_[i];
     }
@@ -433,13 +512,19 @@ public:
         inv.comm_ = comm_;
         inv.map_.assign(map_.size(), 0);
         for (size_t i = 0; i < map_.size(); ++i) inv.map_[map_[i]] = (int)i;
+        inv.stale_ = true;
     }
     // firstMap := this o firstMap (dist_map/impl.hpp Extend)
-    void Extend(DistMap& firstMap) const { Translate(firstMap.map_); }
+    void Extend(DistMap& firstMap) const {
+        Translate(firstMap.map_);
+        firstMap.stale_ = true;
+    }
 
 private:
     mpi::Comm comm_;
     std::vector<int> map_;
+    mutable detail::DeviceBuffer dev_;
+    mutable bool stale_ = true;
 };
 
 // ---------------------------------------------------------------------------
@@ -583,6 +668,8 @@ inline void NestedDissection(const DistGraph&, DistMap&, DistSeparatorTree&, Dis
 // ---------------------------------------------------------------------------
 // front tree (numeric/dist_symm_front_tree)
 // ---------------------------------------------------------------------------
+// SymmFrontType and its predicates and conversions
+// (numeric/dist_symm_front_tree/decl.hpp:17-168)
 enum SymmFrontType {
     SYMM_1D, SYMM_2D,
     LDL_1D, LDL_2D,
@@ -601,10 +688,49 @@ inline bool BlockFactorization(SymmFrontType t) {
     return t == BLOCK_LDL_1D || t == BLOCK_LDL_2D || t == BLOCK_LDL_INTRAPIV_1D ||
            t == BLOCK_LDL_INTRAPIV_2D;
 }
+inline bool SelInvFactorization(SymmFrontType t) {
+    return t == LDL_SELINV_1D || t == LDL_SELINV_2D || t == LDL_INTRAPIV_SELINV_1D ||
+           t == LDL_INTRAPIV_SELINV_2D;
+}
 inline bool PivotedFactorization(SymmFrontType t) {
     return t == LDL_INTRAPIV_1D || t == LDL_INTRAPIV_2D || t == LDL_INTRAPIV_SELINV_1D ||
            t == LDL_INTRAPIV_SELINV_2D || t == BLOCK_LDL_INTRAPIV_1D ||
            t == BLOCK_LDL_INTRAPIV_2D;
+}
+// the 1D and 2D members of each pair are adjacent, 1D first
+inline SymmFrontType ConvertTo2d(SymmFrontType t) {
+    if (t < SYMM_1D || t > BLOCK_LDL_INTRAPIV_2D) throw LogicError("Invalid front type");
+    return SymmFrontType(t | 1);
+}
+inline SymmFrontType ConvertTo1d(SymmFrontType t) {
+    if (t < SYMM_1D || t > BLOCK_LDL_INTRAPIV_2D) throw LogicError("Invalid front type");
+    return SymmFrontType(t & ~1);
+}
+inline SymmFrontType AppendSelInv(SymmFrontType t) {
+    switch (t) {
+        case LDL_1D: return LDL_SELINV_1D;
+        case LDL_2D: return LDL_SELINV_2D;
+        case LDL_INTRAPIV_1D: return LDL_INTRAPIV_SELINV_1D;
+        case LDL_INTRAPIV_2D: return LDL_INTRAPIV_SELINV_2D;
+        default: throw LogicError("Sel-inv does not make sense for this type");
+    }
+}
+inline SymmFrontType RemoveSelInv(SymmFrontType t) {
+    switch (t) {
+        case LDL_SELINV_1D: return LDL_1D;
+        case LDL_SELINV_2D: return LDL_2D;
+        case LDL_INTRAPIV_SELINV_1D: return LDL_INTRAPIV_1D;
+        case LDL_INTRAPIV_SELINV_2D: return LDL_INTRAPIV_2D;
+        default: throw LogicError("This type did not involve selective inversion");
+    }
+}
+// the factorization LDL runs before converting to the requested type
+// (dist_symm_front_tree/decl.hpp:158-168)
+inline SymmFrontType InitialFactorType(SymmFrontType t) {
+    if (Unfactored(t)) throw LogicError("Front type does not require factorization");
+    if (BlockFactorization(t)) return ConvertTo2d(t);
+    if (PivotedFactorization(t)) return LDL_INTRAPIV_2D;
+    return LDL_2D;
 }
 
 // Arithmetic of the numeric factorization (no counterpart in the reference,
@@ -619,14 +745,32 @@ struct SymmFront {
     Matrix<int> piv;
     mutable Matrix<F> work;
 };
+// A distributed front (dist_symm_front_tree/decl.hpp:187-204). At world size
+// 1 there is one, the leaf of the distributed tree, which is the local root:
+// front1dL and front2dL both hold that front.
+template <typename F>
+struct DistSymmFront {
+    Matrix<F> front1dL, front2dL;
+    Matrix<F> diag, subdiag;
+    Matrix<int> piv;
+};
+
+template <typename F> class DistSymmFrontTree;
+template <typename F> void ChangeFrontType(DistSymmFrontTree<F>& L, SymmFrontType frontType);
+template <typename F>
+void LDL(DistSymmInfo& info, DistSymmFrontTree<F>& L, SymmFrontType newType = LDL_2D,
+         FactorPrecision precision = FACTOR_FP64);
 
 template <typename F>
 class DistSymmFrontTree {
 public:
     bool isHermitian = false;
     SymmFrontType frontType = SYMM_2D;
-    // host mirrors of the device fronts, filled by MirrorFronts()
+    // Host mirrors of the device fronts, filled by MirrorFronts(). After
+    // ChangeFrontType(L, SYMM_1D/SYMM_2D) on a factored tree they are the
+    // values the next LDL factors (edits to frontL go to the device then).
     mutable std::vector<SymmFront<F>> localFronts;
+    mutable std::vector<DistSymmFront<F>> distFronts;
 
     DistSymmFrontTree() = default;
     DistSymmFrontTree(const DistSparseMatrix<F>& A, const DistMap& reordering,
@@ -655,10 +799,13 @@ public:
         vals_.upload(A.Values().data(), A.Values().size() * sizeof(F));
         isHermitian = conjugate;
         frontType = SYMM_2D;
+        hostFronts_ = false;
         localFronts.clear();
+        distFronts.clear();
     }
 
-    // Copies every front (frontL and diag) of the factored tree to the host.
+    // Copies every front (frontL, diag) of the factored tree to the host;
+    // distFronts[0] mirrors the root.
     void MirrorFronts() const {
         const int num = (int)sym_nodes();
         localFronts.assign(num, SymmFront<F>());
@@ -675,6 +822,20 @@ public:
                                                    : localFronts[s].diag.Buffer()),
                           "MirrorFronts");
         }
+        distFronts.assign(1, DistSymmFront<F>());
+        if (num) {
+            distFronts[0].front1dL = localFronts.back().frontL;
+            distFronts[0].front2dL = localFronts.back().frontL;
+            distFronts[0].diag = localFronts.back().diag;
+        }
+    }
+
+    // Sends the host fronts (localFronts[s].frontL, unfactored values) to
+    // the plan; the next LDL factors them instead of A's values.
+    void PushFronts() const {
+        for (int s = 0; s < (int)localFronts.size(); ++s)
+            detail::check(cliqg_plan_set_front(plan_.get(), s, localFronts[s].frontL.LockedBuffer()),
+                          "PushFronts");
     }
 
     void MemoryInfo(double& numLocalEntries, double& minLocalEntries, double& maxLocalEntries,
@@ -717,6 +878,9 @@ public:
     }
 
 private:
+    template <typename G> friend void ChangeFrontType(DistSymmFrontTree<G>&, SymmFrontType);
+    template <typename G>
+    friend void LDL(DistSymmInfo&, DistSymmFrontTree<G>&, SymmFrontType, FactorPrecision);
     size_t sym_nodes() const {
         int num = 0;
         detail::check(cliqg_symbolic_num_nodes(sym_.get(), &num, nullptr), "front count");
@@ -725,26 +889,60 @@ private:
     std::shared_ptr<cliqg_symbolic_s> sym_;
     std::shared_ptr<cliqg_plan_s> plan_;
     detail::DeviceBuffer vals_;
+    bool hostFronts_ = false;   // localFronts hold the values the next LDL factors
 };
 
-// (numeric/ldl.hpp:57-75) Factors the tree in place. LDL_* and
-// LDL_SELINV_* (selective inversion is a no-op at world size 1) give unit-L
-// fronts; BLOCK_LDL_* inverts the diagonal blocks; intra-front pivoting is
-// not offered. A tree is refactored by calling Initialize again.
+// (numeric/change_front_type.hpp:26-109) At world size 1 the 1D and 2D
+// layouts of the single distributed front are the same storage, and
+// selective inversion touches distributed fronts s >= 1 only, of which there
+// are none: those changes relabel the tree. Resetting a factored tree to
+// SYMM_1D/SYMM_2D (tests/Refactor.cpp:150) hands its fronts to the caller
+// as localFronts, which the next LDL factors; the reference has no such
+// branch and throws "Unavailable front type change" there.
 template <typename F>
-inline void LDL(DistSymmInfo& info, DistSymmFrontTree<F>& L, SymmFrontType newType = LDL_2D,
-                FactorPrecision precision = FACTOR_FP64) {
+inline void ChangeFrontType(DistSymmFrontTree<F>& L, SymmFrontType frontType) {
+    const SymmFrontType old = L.frontType;
+    if (frontType == old) return;
+    if (frontType == ConvertTo2d(old) || frontType == ConvertTo1d(old)) {
+        L.frontType = frontType;
+    } else if (SelInvFactorization(frontType) &&
+               ConvertTo2d(frontType) == ConvertTo2d(AppendSelInv(old))) {
+        L.frontType = frontType;
+    } else if (Unfactored(frontType) && !Unfactored(old)) {
+        if (L.localFronts.empty()) L.MirrorFronts();
+        L.hostFronts_ = true;
+        L.frontType = frontType;
+    } else {
+        throw LogicError("Unavailable front type change");
+    }
+}
+
+// (numeric/ldl.hpp:57-75) Factors the tree in place: the initial
+// factorization (InitialFactorType) in the requested precision, then the
+// conversion to the requested front type. LDL_* and LDL_SELINV_* give
+// unit-L fronts, BLOCK_LDL_* inverted diagonal blocks. A tree is refactored
+// by calling Initialize again (new values) or ChangeFrontType to SYMM_*
+// (the edited localFronts).
+template <typename F>
+inline void LDL(DistSymmInfo& info, DistSymmFrontTree<F>& L, SymmFrontType newType,
+                FactorPrecision precision) {
     (void)info;
     if (!Unfactored(L.frontType)) throw LogicError("Matrix is already factored");
-    if (Unfactored(newType)) throw LogicError("Invalid front type for LDL");
-    if (PivotedFactorization(newType))
+    ChangeFrontType(L, SYMM_2D);
+    const SymmFrontType initial = InitialFactorType(newType);
+    if (PivotedFactorization(initial))
         throw LogicError("Intra-front pivoting (Bunch-Kaufman) is not available in this build");
-    const cliqg_front_type ft = BlockFactorization(newType) ? CLIQG_FRONT_BLOCK_LDL
+    if (L.hostFronts_) L.PushFronts();
+    const cliqg_front_type ft = BlockFactorization(initial) ? CLIQG_FRONT_BLOCK_LDL
                                                             : CLIQG_FRONT_LDL;
     detail::check(cliqg_plan_factor(L.Plan(), L.DeviceValues(),
                                     precision == FACTOR_FP32 ? CLIQG_F32 : CLIQG_F64, ft),
                   "LDL");
-    L.frontType = newType;
+    L.hostFronts_ = false;
+    L.localFronts.clear();
+    L.distFronts.clear();
+    L.frontType = initial;
+    ChangeFrontType(L, newType);
 }
 
 // ---------------------------------------------------------------------------
@@ -758,25 +956,27 @@ public:
                       const DistMultiVec<F>& X) {
         Pull(inverseMap, info, X);
     }
-    // X_nodal(p, j) = X(inverseMap[p], j)   (impl.hpp:67-217)
+    // X_nodal(p, j) = X(inverseMap[p], j)   (impl.hpp:67-217), on the device
     void Pull(const DistMap& inverseMap, const DistSymmInfo&, const DistMultiVec<F>& X) {
         h_ = X.Height();
         w_ = X.Width();
         if (inverseMap.NumSources() != h_) throw LogicError("map and vector do not match");
-        std::vector<F> tmp((size_t)h_ * w_);
-        for (int j = 0; j < w_; ++j)
-            for (int p = 0; p < h_; ++p)
-                tmp[p + (size_t)j * h_] = X.GetLocal(inverseMap.GetLocal(p), j);
-        buf_.upload(tmp.data(), tmp.size() * sizeof(F));
+        buf_.resize((size_t)h_ * w_ * sizeof(F));
+        if (h_ && w_)
+            detail::check(cliqg_permute_rows(h_, w_, inverseMap.DeviceMap(), 0,
+                                             X.LockedDeviceBuffer(), X.LDim(), buf_.get(), h_,
+                                             detail::Field<F>::value, nullptr),
+                          "Pull");
     }
-    // X(inverseMap[p], j) = X_nodal(p, j)   (impl.hpp:219-356)
+    // X(inverseMap[p], j) = X_nodal(p, j)   (impl.hpp:219-356), on the device
     void Push(const DistMap& inverseMap, const DistSymmInfo&, DistMultiVec<F>& X) const {
-        std::vector<F> tmp((size_t)h_ * w_);
-        buf_.download(tmp.data(), tmp.size() * sizeof(F));
-        X.ResizeTo(h_, w_);
-        for (int j = 0; j < w_; ++j)
-            for (int p = 0; p < h_; ++p)
-                X.SetLocal(inverseMap.GetLocal(p), j, tmp[p + (size_t)j * h_]);
+        if (inverseMap.NumSources() != h_) throw LogicError("map and vector do not match");
+        if (X.Height() != h_ || X.Width() != w_) X.ResizeTo(h_, w_);
+        if (h_ && w_)
+            detail::check(cliqg_permute_rows(h_, w_, inverseMap.DeviceMap(), 1, buf_.get(), h_,
+                                             X.DeviceBuffer(), X.LDim(),
+                                             detail::Field<F>::value, nullptr),
+                          "Push");
     }
     int Height() const { return h_; }
     int Width() const { return w_; }
@@ -837,12 +1037,16 @@ inline void solve_nodal(const DistSymmFrontTree<F>& L, void* buf, int h, int w) 
 
 // (numeric/solve.hpp:40-92) 1D front types solve against a
 // DistNodalMultiVec, 2D types against a DistNodalMatrix; block front types
-// skip the diagonal solve inside the engine.
+// skip the diagonal solve inside the engine. As in the reference, 1D block
+// fronts are rejected by the distributed sweep (lower_solve/dist.hpp:54-56).
 template <typename F>
 inline void Solve(const DistSymmInfo& info, const DistSymmFrontTree<F>& L,
                   DistNodalMultiVec<F>& X) {
     (void)info;
     if (!FrontsAre1d(L.frontType)) throw LogicError("Invalid front type for 1D solve");
+    if (Unfactored(L.frontType)) throw LogicError("Nonsensical front type for solve");
+    if (L.frontType == BLOCK_LDL_1D || L.frontType == BLOCK_LDL_INTRAPIV_1D)
+        throw LogicError("1D block solves not yet supported by this routine");
     detail::solve_nodal(L, X.DeviceBuffer(), X.Height(), X.Width());
 }
 template <typename F>
@@ -854,25 +1058,22 @@ inline void Solve(const DistSymmInfo& info, const DistSymmFrontTree<F>& L,
 }
 
 // FP64 iterative refinement of A X = B (no reference counterpart): X is
-// overwritten with the refined solution; returns the refinement report.
+// overwritten with the refined solution (device-resident: B and X stay in
+// HBM, nothing crosses to the host); returns the refinement report.
 template <typename F>
 inline cliqg_refine_info Refine(const DistSymmFrontTree<F>& L, const DistMultiVec<F>& B,
                                 DistMultiVec<F>& X, double tol = 1e-10, int maxIters = 20) {
-    const size_t bytes = (size_t)B.Height() * B.Width() * sizeof(F);
-    detail::DeviceBuffer db, dx;
-    db.upload(B.LockedBuffer(), bytes);
-    dx.resize(bytes);
+    if (X.Height() != B.Height() || X.Width() != B.Width()) X.ResizeTo(B.Height(), B.Width());
     cliqg_refine_info r{};
-    const int ld = std::max(B.Height(), 1);
-    const cliqg_status st = cliqg_plan_refine(L.Plan(), db.get(), ld, dx.get(), ld, B.Width(),
-                                              tol, maxIters, &r);
-    X.ResizeTo(B.Height(), B.Width());
-    dx.download(X.Buffer(), bytes);
+    const cliqg_status st = cliqg_plan_refine(L.Plan(), B.LockedDeviceBuffer(), B.LDim(),
+                                              X.DeviceBuffer(), X.LDim(), B.Width(), tol, maxIters,
+                                              &r);
     if (st != CLIQG_ERR_NONCONVERGED) detail::check(st, "Refine");
     return r;
 }
 
-// (numeric/multiply.hpp:27-166) Y := alpha A X + beta Y on the device.
+// (numeric/multiply.hpp:27-166) Y := alpha A X + beta Y on the device (A's
+// CSR and both multi-vectors stay in HBM).
 template <typename T>
 inline void Multiply(T alpha, const DistSparseMatrix<T>& A, const DistMultiVec<T>& X, T beta,
                      DistMultiVec<T>& Y) {
@@ -881,21 +1082,13 @@ inline void Multiply(T alpha, const DistSparseMatrix<T>& A, const DistMultiVec<T
     if (std::imag(std::complex<double>(alpha)) != 0 || std::imag(std::complex<double>(beta)) != 0)
         throw LogicError("Multiply supports real alpha and beta");
     const int N = A.Height(), k = X.Width();
-    const cliq::DistGraph& g = A.LockedDistGraph();
-    detail::DeviceBuffer off, cols, vals, dx, dy;
-    off.upload(g.Offsets().data(), g.Offsets().size() * sizeof(int64_t));
-    cols.upload(g.Targets().data(), g.Targets().size() * sizeof(int));
-    vals.upload(A.Values().data(), A.Values().size() * sizeof(T));
-    dx.upload(X.LockedBuffer(), (size_t)N * k * sizeof(T));
-    dy.upload(Y.LockedBuffer(), (size_t)N * k * sizeof(T));
-    const int ld = std::max(N, 1);
-    detail::check(cliqg_spmv(N, (const int64_t*)off.get(), (const int*)cols.get(), vals.get(),
+    if (N == 0 || k == 0) return;
+    detail::check(cliqg_spmv(N, A.DeviceRowOffsets(), A.DeviceCols(), A.DeviceValues(),
                              detail::Field<T>::value, std::real(std::complex<double>(alpha)),
-                             dx.get(), ld, std::real(std::complex<double>(beta)), dy.get(), ld, k,
+                             X.LockedDeviceBuffer(), X.LDim(),
+                             std::real(std::complex<double>(beta)), Y.DeviceBuffer(), Y.LDim(), k,
                              nullptr),
                   "Multiply");
-    detail::check(cliqg_device_synchronize(), "Multiply");
-    dy.download(Y.Buffer(), (size_t)N * k * sizeof(T));
 }
 
 // Black-box solve (numeric/solve.hpp:94-116) on a grid graph: natural

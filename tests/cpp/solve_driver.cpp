@@ -10,6 +10,38 @@
 #include "clique.hpp"
 using namespace cliq;
 
+// The unfactored fronts of A, as DistSymmFrontTree::Initialize scatters them
+// (reference dist_symm_front_tree/impl.hpp:241-297), times `scale`: the
+// values a Refactor.cpp-style caller writes into localFronts.
+template <typename F>
+void assemble_fronts(const DistSparseMatrix<F>& A, const DistMap& map,
+                     const DistSeparatorTree& sepTree, const DistSymmInfo& info, F scale,
+                     std::vector<SymmFront<F>>& fronts) {
+    for (size_t s = 0; s < info.localNodes.size(); ++s) {
+        const SymmNodeInfo& nd = info.localNodes[s];
+        const int n = nd.size, m = n + (int)nd.lowerStruct.size();
+        Matrix<F>& L = fronts[s].frontL;
+        L.Resize(m, n);
+        const std::vector<int>& inds = sepTree.localSepsAndLeaves[s]->inds;
+        for (int t = 0; t < n; ++t) {
+            const int i = inds[t];
+            for (int e = A.LocalEntryOffset(i); e < A.LocalEntryOffset(i) + A.NumConnections(i); ++e) {
+                const int target = map.GetLocal(A.Col(e));
+                if (target < nd.off + t) continue;
+                int row;
+                if (target < nd.off + n) {
+                    row = target - nd.off;
+                } else {
+                    auto it = std::lower_bound(nd.origLowerStruct.begin(), nd.origLowerStruct.end(),
+                                               target);
+                    row = nd.origLowerRelInds[it - nd.origLowerStruct.begin()];
+                }
+                L.Set(row, t, scale * A.Value(e));
+            }
+        }
+    }
+}
+
 template <typename F>
 int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
     mpi::Comm comm = mpi::COMM_WORLD;
@@ -58,13 +90,51 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
     DistMultiVec<F> Xc(N, numRhs, comm);
     double relErr = 0;
     bool wrongKindThrew = false;
-    if (!fp32 && FrontsAre1d(ftype)) {
+    bool checks = true;
+    if (!fp32 && (ftype == BLOCK_LDL_1D)) {
+        // as in the reference, 1D block fronts cannot be solved against a
+        // DistNodalMultiVec (lower_solve/dist.hpp:54-56); the caller
+        // converts them to 2D and solves against a DistNodalMatrix
+        DistNodalMultiVec<F> V(inverseMap, info, Y);
+        try { Solve(info, frontTree, V); } catch (const LogicError&) { wrongKindThrew = true; }
+        ChangeFrontType(frontTree, BLOCK_LDL_2D);
+        DistNodalMatrix<F> YNodal(inverseMap, info, Y);
+        Solve(info, frontTree, YNodal);
+        YNodal.Push(inverseMap, info, Xc);
+    } else if (!fp32 && FrontsAre1d(ftype)) {
         DistNodalMultiVec<F> YNodal;
         YNodal.Pull(inverseMap, info, Y);
         Solve(info, frontTree, YNodal);
         YNodal.Push(inverseMap, info, Xc);
         DistNodalMatrix<F> M(inverseMap, info, Y);
         try { Solve(info, frontTree, M); } catch (const LogicError&) { wrongKindThrew = true; }
+        // device norms of the pushed (device-current) solution against the
+        // host loop
+        std::vector<typename Base<F>::type> dn;
+        Norms(Xc, dn);
+        for (int j = 0; j < numRhs; ++j) {
+            double h = 0;
+            for (int i = 0; i < N; ++i) h += std::norm(Xc.GetLocal(i, j));
+            checks = checks && std::abs(std::sqrt(h) - dn[j]) <= 1e-12 * std::sqrt(h);
+        }
+        // Refactor.cpp:150-160: reset the tree, write the fronts, factor
+        // again. The fronts written are those of 2 A, so X/2 must come out.
+        ChangeFrontType(frontTree, SYMM_2D);
+        checks = checks && frontTree.localFronts.size() == info.localNodes.size() &&
+                 frontTree.distFronts.size() == 1;
+        assemble_fronts(A, map, sepTree, info, F(2), frontTree.localFronts);
+        LDL(info, frontTree, ftype);
+        DistNodalMultiVec<F> Y2(inverseMap, info, Y);
+        Solve(info, frontTree, Y2);
+        DistMultiVec<F> X2(N, numRhs, comm);
+        Y2.Push(inverseMap, info, X2);
+        double worst = 0;
+        for (int j = 0; j < numRhs; ++j)
+            for (int i = 0; i < N; ++i)
+                worst = std::max(worst, std::abs(F(2) * X2.GetLocal(i, j) - Xc.GetLocal(i, j)) /
+                                            std::abs(Xc.GetLocal(i, j)));
+        std::printf("refactor from host fronts of 2A: max |2 x2 - x| / |x| = %.3e\n", worst);
+        checks = checks && worst <= 1e-10;
     } else if (!fp32) {
         // 2D front types solve against a DistNodalMatrix (solve.hpp:67-92)
         DistNodalMatrix<F> YNodal(inverseMap, info, Y);
@@ -96,9 +166,9 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
         threw = true;
     }
     const double tol = fp32 ? 1e-8 : 1e-10;
-    if (relErr > tol || !threw || !wrongKindThrew) {
-        std::printf("FAILED (relative error %.3e, refactor guard %d, 1D/2D guard %d)\n", relErr,
-                    (int)threw, (int)wrongKindThrew);
+    if (relErr > tol || !threw || !wrongKindThrew || !checks) {
+        std::printf("FAILED (relative error %.3e, refactor guard %d, 1D/2D guard %d, checks %d)\n",
+                    relErr, (int)threw, (int)wrongKindThrew, (int)checks);
         return 1;
     }
     std::printf("PASSED\n");

@@ -445,3 +445,39 @@ def test_tuning_switches_keep_results(device):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
     assert float(r.stdout.strip().splitlines()[-1]) <= 1e-13, r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_staged_fronts_replace_assembled_values(device, precision):
+    """Fronts written from the host before a factorization (the reference's
+    tests/Refactor.cpp writes localFronts[s].frontL and factors again) are
+    factored in place of A's values: staging every front of 2A (the oracle's
+    assembled fronts, doubled) gives the solution of 2A x = b, x/2 (scaling
+    by two is exact, so to rounding of the division only); the next
+    factorization without staging is A's again, bit for bit."""
+    ip, ind, v, dims = problem("helm", 10)
+    N = ip.size - 1
+    sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=16)
+    ft = cq.DistSymmFrontTree(sym, ip, ind, is_complex=True)
+    vals = torch.tensor(v, device="cuda:0")
+    t = oracle.Tree(*dims, ip, ind, cutoff=16)
+    f = oracle.Factor(t, is_complex=True)
+    f.initialize(ip, ind, v)        # assembled, not factored
+    b = np.random.default_rng(41).standard_normal((N, 2)) + 0j
+
+    def solve():
+        y = to_dev(b)
+        ft.solve(y)
+        return y.cpu().numpy()
+
+    ft.factor(vals, precision=precision)
+    y1 = solve()
+    for s in range(sym.num_nodes):
+        ft.set_front(s, 2.0 * f.front(s)[0])
+    ft.factor(vals, precision=precision)
+    y2 = solve()
+    ft.factor(vals, precision=precision)
+    y3 = solve()
+    assert rel_diff(2.0 * y2, y1) <= 1e-12
+    assert np.array_equal(y3, y1)
