@@ -50,6 +50,7 @@ def lib():
         L.or_ldl.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_solve.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int]
         L.or_plant.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.or_get_pivots.argtypes = [vp, ctypes.c_int, dp, ip]
         L.or_blas_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.or_ldl_blas.argtypes = [vp, ctypes.c_int]
         L.or_solve_blas.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int]
@@ -186,10 +187,22 @@ class Factor:
         if r != 0:
             raise RuntimeError(f"or_initialize failed ({r})")
 
-    def ldl(self, nb=96, block=False, nthreads=0):
-        r = lib().or_ldl(self._h, nb, int(block), nthreads)
+    def ldl(self, nb=96, block=False, nthreads=0, pivoted=False):
+        """LocalLDL: FrontLDL (default), FrontBlockLDL (block) or
+        FrontLDLIntraPiv with Bunch-Kaufman pivoting (pivoted)."""
+        mode = 2 if pivoted else int(bool(block))
+        r = lib().or_ldl(self._h, nb, mode, nthreads)
         if r != 0:
             raise ZeroDivisionError(f"oracle LDL hit a zero pivot ({r})")
+
+    def pivots(self, s):
+        """(subdiag, piv) of front s of a pivoted factor."""
+        n = self.tree.node(s)["size"]
+        dt = np.complex128 if self.is_complex else np.float64
+        sub = np.zeros(max(n, 1), dtype=dt)
+        piv = np.zeros(max(n, 1), dtype=np.int32)
+        lib().or_get_pivots(self._h, s, _dp(sub.view(np.float64)), _ip(piv))
+        return sub[:n], piv[:n]
 
     def ldl_blas(self, nb=96):
         """LocalLDL with FrontLDL's dense calls made through the CPU BLAS

@@ -6,6 +6,7 @@
 
 #include <complex.h>
 #include <dlfcn.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -336,8 +337,11 @@ struct or_factor {
     const or_tree* t;
     int is_complex, conjugate, block, factored;
     int plant_front, plant_panel;   /* planted mistake of the next LDL (-1: none) */
+    int pivoted;   /* LDL_INTRAPIV: Bunch-Kaufman inside each front's ATL */
     void** L;      /* per front m x n */
     void** diag;   /* per front n     */
+    void** sub;    /* per front n: subdiagonal of the 2x2 pivots (0 after 1x1) */
+    int** piv;     /* per front n: the ATL permutation                        */
 };
 
 /* A multithreaded BLAS loaded at run time for the CPU-baseline path
@@ -363,23 +367,27 @@ static or_blas g_blas;
 #define T double
 #define SUF d
 #define BLP d
+#define ABSV(x) fabs(x)
 #define CONJ(x) (x)
 #include "oracle_num.inc"
 #include "oracle_blas.inc"
 #undef T
 #undef SUF
 #undef BLP
+#undef ABSV
 #undef CONJ
 
 #define T double complex
 #define SUF z
 #define BLP z
+#define ABSV(x) cabs(x)
 #define CONJ(x) (f->conjugate ? conj(x) : (x))
 #include "oracle_num.inc"
 #include "oracle_blas.inc"
 #undef T
 #undef SUF
 #undef BLP
+#undef ABSV
 #undef CONJ
 
 int or_blas_load(const char* path, const char* prefix, int threads) {
@@ -413,6 +421,7 @@ int or_blas_load(const char* path, const char* prefix, int threads) {
 int or_ldl_blas(or_factor* f, int nb) {
     if (!g_blas.loaded || nb < 1) return -1;
     f->block = 0;
+    f->pivoted = 0;
     int r = f->is_complex ? ldl_blas_z(f, nb) : ldl_blas_d(f, nb);
     if (r == 0) f->factored = 1;
     return r;
@@ -431,13 +440,17 @@ or_factor* or_factor_create(const or_tree* t, int is_complex, int conjugate) {
     f->plant_front = f->plant_panel = -1;
     f->L = (void**)calloc((size_t)t->num, sizeof(void*));
     f->diag = (void**)calloc((size_t)t->num, sizeof(void*));
+    f->sub = (void**)calloc((size_t)t->num, sizeof(void*));
+    f->piv = (int**)calloc((size_t)t->num, sizeof(int*));
     return f;
 }
 
 void or_factor_free(or_factor* f) {
     if (!f) return;
-    for (int s = 0; s < f->t->num; ++s) { free(f->L[s]); free(f->diag[s]); }
-    free(f->L); free(f->diag); free(f);
+    for (int s = 0; s < f->t->num; ++s) {
+        free(f->L[s]); free(f->diag[s]); free(f->sub[s]); free(f->piv[s]);
+    }
+    free(f->L); free(f->diag); free(f->sub); free(f->piv); free(f);
 }
 
 int or_initialize(or_factor* f, int N, const long* rowoff, const int* cols, const double* vals) {
@@ -451,8 +464,9 @@ int or_ldl(or_factor* f, int nb, int block, int nthreads) {
     (void)nthreads;
 #endif
     if (nb < 1) return -1;
-    f->block = block;
-    int r = f->is_complex ? ldl_z(f, nb, block) : ldl_d(f, nb, block);
+    f->block = block == 1;
+    f->pivoted = block == 2;
+    int r = f->is_complex ? ldl_z(f, nb, f->block) : ldl_d(f, nb, f->block);
     if (r == 0) f->factored = 1;
     return r;
 }
@@ -465,6 +479,14 @@ void or_plant(or_factor* f, int front, int panel) {
 int or_solve(or_factor* f, double* x, int ldx, int k) {
     if (!f->factored) return -1;
     return f->is_complex ? solve_z(f, x, ldx, k) : solve_d(f, x, ldx, k);
+}
+
+int or_get_pivots(const or_factor* f, int s, double* sub, int* piv) {
+    const size_t n = (size_t)f->t->nodes[s].size;
+    const size_t es = f->is_complex ? 2 * sizeof(double) : sizeof(double);
+    if (sub && n && f->sub[s]) memcpy(sub, f->sub[s], es * n);
+    if (piv && n && f->piv[s]) memcpy(piv, f->piv[s], sizeof(int) * n);
+    return 0;
 }
 
 int or_get_front(const or_factor* f, int s, double* L, double* diag) {

@@ -13,6 +13,8 @@
  *   or_initialize      numeric/dist_symm_front_tree/impl.hpp:241-297
  *   or_ldl             numeric/ldl/local.hpp:25-122 (LocalLDL),
  *                      numeric/ldl/local_front.hpp:29-71 (FrontLDL),
+ *                      numeric/ldl/local_front.hpp:73-94 (FrontLDLIntraPiv;
+ *                      Bunch-Kaufman "A" of Elemental's El::LDL restated),
  *                      numeric/ldl/local_front_block.hpp:25-73 (FrontBlockLDL)
  *   or_solve           numeric/solve.hpp:40-65, lower_solve/local.hpp:41-119,
  *                      202-277, lower_solve/local_front.hpp:35-82,
@@ -65,8 +67,13 @@ void or_factor_free(or_factor* f);
 /* DistSymmFrontTree::Initialize from a CSR matrix in the original ordering */
 int or_initialize(or_factor* f, int N, const long* rowoff, const int* cols,
                   const double* vals);
-/* LocalLDL; block != 0 selects FrontBlockLDL; nthreads <= 0: default */
+/* LocalLDL; block = 1 selects FrontBlockLDL, block = 2 FrontLDLIntraPiv
+ * (Bunch-Kaufman inside each front's ATL, ldl/local_front.hpp:73-94);
+ * nthreads <= 0: default */
 int or_ldl(or_factor* f, int nb, int block, int nthreads);
+/* front s of a pivoted factor: D's subdiagonal (n entries, 0 after a 1x1
+ * pivot) and the permutation p, (P ATL P^T)(i,j) = ATL(p[i], p[j]) */
+int or_get_pivots(const or_factor* f, int s, double* sub, int* piv);
 /* Test hook: the next or_ldl leaves out the trailing update of panel
  * `panel` of front `front` (front < 0 clears it). Used only to show that the
  * tests' tolerances detect such a mistake. */

@@ -10,6 +10,7 @@ and scipy's sparse direct solver.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import oracle
@@ -340,3 +341,128 @@ def test_blas_baseline_matches_oracle(kind, n, cut, cplx, conj):
     xa = ref.solve_nodal(b)
     xb = f.solve_nodal_blas(b)
     assert rel_diff(xb, xa) <= 1e-12
+
+
+def indefinite_box(dims, seed=0, cplx=False, herm=False):
+    """A 7-point operator with a weak diagonal of random sign: indefinite, so
+    Bunch-Kaufman takes swaps and 2x2 pivots."""
+    ip, ind, v, dims = P.stencil7(*dims, 1.0, -1.0)
+    rng = np.random.default_rng(seed)
+    N = ip.size - 1
+    rows = np.repeat(np.arange(N), np.diff(ip))
+    v = v.astype(np.complex128) if cplx else v.copy()
+    diag = rows == ind
+    v[diag] = rng.uniform(-0.6, 0.6, N) + (1j * rng.uniform(-0.3, 0.3, N) if cplx and not herm else 0)
+    if herm:
+        # complex couplings, Hermitian: (j,i) holds the conjugate of (i,j)
+        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, ind.size))
+        A = sp.csr_matrix((v * np.where(diag, 1, ph), ind, ip), shape=(N, N))
+        A = sp.triu(A, 1) + sp.triu(A, 1).conj().T + sp.diags(A.diagonal().real)
+        A = A.tocsr()
+        A.sort_indices()
+        return A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, dims
+    return ip, ind, v, dims
+
+
+def _dense_nodal(ip, ind, v, t):
+    A = P.to_scipy(ip, ind, v).toarray()
+    mp = t.map()
+    Ap = np.zeros_like(A)
+    Ap[np.ix_(mp, mp)] = A
+    return Ap
+
+
+def _ldl_parts(L, d, sub, conj):
+    n = L.shape[1]
+    Lu = np.tril(L[:n, :n], -1) + np.eye(n)
+    D = np.diag(d[:n]).astype(L.dtype)
+    for k in range(n - 1):
+        if sub[k] != 0:
+            D[k + 1, k] = sub[k]
+            D[k, k + 1] = np.conj(sub[k]) if conj else sub[k]
+    return Lu, D
+
+
+def test_bunch_kaufman_matches_lapack_sytrf_real():
+    """One front holding a whole real symmetric indefinite matrix: the
+    oracle's Bunch-Kaufman (the "A" rule) gives LAPACK dsytrf's pivots, D and
+    L (scipy.linalg.ldl), an independent implementation of the same rule."""
+    import scipy.linalg as sla
+    ip, ind, v, dims = indefinite_box((6, 5, 4), seed=3)
+    t = oracle.Tree(*dims, ip, ind, cutoff=1000)
+    assert t.num == 1
+    f = oracle.Factor(t)
+    f.initialize(ip, ind, v)
+    f.ldl(pivoted=True)
+    L, d = f.front(0)
+    sub, piv = f.pivots(0)
+    Ap = _dense_nodal(ip, ind, v, t)
+    lu, dd, perm = sla.ldl(Ap, lower=True, hermitian=False)
+    assert np.array_equal(piv, perm)
+    assert (sub != 0).sum() > 3           # 2x2 pivots were taken
+    assert not np.array_equal(piv, np.arange(piv.size))
+    Lu, D = _ldl_parts(L, d, sub, False)
+    assert np.abs(D - dd).max() <= 1e-12 * np.abs(dd).max()
+    assert np.abs(Lu - lu[perm]).max() <= 1e-12 * np.abs(lu).max()
+
+
+@pytest.mark.parametrize("cplx,herm", [(False, False), (True, False), (True, True)])
+def test_bunch_kaufman_reconstructs_the_front(cplx, herm):
+    """P ATL P^T = L D op(L)^T for real, complex-symmetric and Hermitian
+    indefinite matrices held in one front."""
+    ip, ind, v, dims = indefinite_box((5, 5, 5), seed=4, cplx=cplx, herm=herm)
+    t = oracle.Tree(*dims, ip, ind, cutoff=1000)
+    f = oracle.Factor(t, is_complex=cplx, conjugate=herm)
+    f.initialize(ip, ind, v)
+    f.ldl(pivoted=True)
+    L, d = f.front(0)
+    sub, piv = f.pivots(0)
+    Ap = _dense_nodal(ip, ind, v, t)
+    Lu, D = _ldl_parts(L, d, sub, herm)
+    R = Lu @ D @ (Lu.conj().T if herm else Lu.T)
+    assert np.abs(R - Ap[np.ix_(piv, piv)]).max() <= 1e-12 * np.abs(Ap).max()
+    assert (sub != 0).sum() > 0
+
+
+@pytest.mark.parametrize("cplx,herm", [(False, False), (True, False), (True, True)])
+def test_pivoted_solve_matches_scipy(cplx, herm):
+    """LDL_INTRAPIV over a whole tree (pivoting inside every front's ATL,
+    PermuteCols / Trsm / QuasiDiagonalSolve / Trrk for ABL and ABR) and the
+    pivoted solves (PermuteRows, QuasiDiagonalSolve, InversePermuteRows)
+    against SuperLU."""
+    import scipy.sparse.linalg as sla
+    ip, ind, v, dims = indefinite_box((9, 8, 7), seed=5, cplx=cplx, herm=herm)
+    t = oracle.Tree(*dims, ip, ind, cutoff=24)
+    assert t.num > 7
+    f = oracle.Factor(t, is_complex=cplx, conjugate=herm)
+    f.initialize(ip, ind, v)
+    f.ldl(pivoted=True)
+    N = ip.size - 1
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((N, 3)) + (1j * rng.standard_normal((N, 3)) if cplx else 0)
+    x = f.solve(b)
+    xr = sla.spsolve(P.to_scipy(ip, ind, v).tocsc(), b)
+    assert rel_diff(x, xr) <= 1e-10
+    assert backward_error(ip, ind, v, x, b).max() <= 1e-13
+    assert sum((f.pivots(s)[0] != 0).sum() for s in range(t.num)) > 0
+
+
+def test_bunch_kaufman_takes_identity_pivots_when_diagonally_dominant():
+    """On the strictly diagonally dominant 30^3 Laplacian (config 1) every
+    pivot is 1x1 without interchange, so the pivoted factor is the
+    unpivoted one."""
+    ip, ind, v, dims = problem("lap", 12)
+    t = oracle.Tree(*dims, ip, ind, cutoff=32)
+    a = oracle.Factor(t)
+    a.initialize(ip, ind, v)
+    a.ldl(pivoted=True)
+    b = oracle.Factor(t)
+    b.initialize(ip, ind, v)
+    b.ldl(nb=96)
+    for s in range(t.num):
+        sub, piv = a.pivots(s)
+        assert np.array_equal(piv, np.arange(piv.size)) and not sub.any()
+        La, da = a.front(s)
+        Lb, db = b.front(s)
+        assert np.abs(La - Lb).max(initial=0) <= 1e-13 * max(np.abs(Lb).max(initial=0), 1)
+        assert np.abs(da - db).max(initial=0) <= 1e-13 * np.abs(db).max(initial=1)
