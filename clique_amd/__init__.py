@@ -28,8 +28,9 @@ from ._lib import CliqgError, build, lib  # noqa: F401
 # SymmFrontType values accepted by LDL (decl.hpp:17-47); 1D/2D coincide at P=1
 LDL_1D = LDL_2D = "ldl"
 BLOCK_LDL_1D = BLOCK_LDL_2D = "block_ldl"
+LDL_INTRAPIV_1D = LDL_INTRAPIV_2D = "ldl_intrapiv"
 
-_FRONT_TYPES = {"ldl": 0, "block_ldl": 1}
+_FRONT_TYPES = {"ldl": 0, "block_ldl": 1, "ldl_intrapiv": 2}
 _PRECISIONS = {"f64": 0, "fp64": 0, "double": 0, "f32": 1, "fp32": 1, "single": 1}
 
 
@@ -257,6 +258,20 @@ class DistSymmFrontTree:
             return L, d[:0]
         return L, d[:n]
 
+    def pivots(self, s):
+        """D's subdiagonal and the pivot permutation of front s of a pivoted
+        factorization (localFronts[s].subdiag and .piv, ldl/local.hpp:108-115):
+        sub[i] = D(i+1, i) (nonzero where a 2x2 pivot starts), and
+        (P ATL P^T)(i, j) = ATL(piv[i], piv[j])."""
+        n = self.sym.node(s)["size"]
+        dt = np.complex128 if self.is_complex else np.float64
+        sub = np.zeros(max(n, 1), dtype=dt)
+        piv = np.zeros(max(n, 1), dtype=np.int32)
+        _lib.check(lib().cliqg_plan_get_front_pivots(
+            self._h, s, sub.ctypes.data_as(ctypes.c_void_p),
+            piv.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return sub[:n], piv[:n]
+
     def set_front(self, s, L):
         """Stage front s's unfactored values (an m x n array, FP64 field type)
         for the next factor(), which uses them in place of A's assembled
@@ -312,11 +327,13 @@ def Multiply(alpha, L: DistSymmFrontTree, X, beta, Y):
 
 
 def SymmetricSolve(nx, ny, nz, indptr, indices, vals, X, conjugate=False, cutoff=128,
-                   precision="f64"):
-    """Black-box solve: dissection, factorization and solve (original ordering)."""
+                   precision="f64", front_type=LDL_INTRAPIV_1D):
+    """Black-box solve: dissection, factorization and solve (original
+    ordering). Like the reference (numeric/solve.hpp:95-117) it factors with
+    intra-front Bunch-Kaufman pivoting (LDL_INTRAPIV_1D) unless told otherwise."""
     import torch
     sym = NaturalNestedDissection(nx, ny, nz, indptr, indices, cutoff)
     is_c = torch.is_complex(X)
     ft = DistSymmFrontTree(sym, indptr, indices, is_complex=is_c, conjugate=conjugate)
-    ft.factor(vals, precision=precision)
+    ft.factor(vals, precision=precision, front_type=front_type)
     return ft.solve(X)

@@ -50,7 +50,7 @@ EXPORTS = [
     "cliqg_plan_refine", "cliqg_plan_adjoint", "cliqg_plan_get_front",
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
     "cliqg_plan_set_graphs", "cliqg_plan_solve_noalloc", "cliqg_plan_set_front",
-    "cliqg_permute_rows", "cliqg_col_norms",
+    "cliqg_permute_rows", "cliqg_col_norms", "cliqg_plan_get_front_pivots",
 ]
 
 
@@ -98,6 +98,7 @@ def lib():
         "cliqg_plan_adjoint": (c_int, [vp, vp, c_int, vp, c_int, c_int, vp, c_int, vp]),
         "cliqg_plan_get_front": (c_int, [vp, c_int, vp, vp]),
         "cliqg_plan_set_front": (c_int, [vp, c_int, vp]),
+        "cliqg_plan_get_front_pivots": (c_int, [vp, c_int, vp, ip]),
         "cliqg_permute_rows": (c_int, [c_int, c_int, vp, c_int, vp, c_int, vp, c_int, c_int, vp]),
         "cliqg_col_norms": (c_int, [c_int, c_int, vp, c_int, c_int, dp, vp]),
         "cliqg_plan_last_factor_ms": (c_int, [vp, dp]),

@@ -389,6 +389,10 @@ cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_dia
     return guarded([&] { P(plan).get_front(s, h_L, h_diag); });
 }
 
+cliqg_status cliqg_plan_get_front_pivots(cliqg_plan plan, int s, void* h_subdiag, int* h_piv) {
+    return guarded([&] { P(plan).get_pivots(s, h_subdiag, h_piv); });
+}
+
 cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms) {
     return guarded([&] {
         require(ms, "null argument");

@@ -138,6 +138,79 @@ struct ZeroLowerDesc {
 // small next to the stores.
 constexpr int ZERO_CHUNK = 16384;
 
+// ---- intra-front Bunch-Kaufman pivoting (pivot.hip) --------------------------
+// Widest panel of the pivoted LDL^T of ATL.
+constexpr int BK_MAX_PANEL = 64;
+
+// One front of a panel step of the pivoted LDL^T of ATL
+// (El::LDL(ATL, dSub, p, conjugate, BUNCH_KAUFMAN_A), ldl/local_front.hpp:86).
+template <class T>
+struct BkDesc {
+    T* A;          // frontL (ld); ATL is rows and columns 0..n-1
+    T* W;          // W = L D of ATL's columns (ld), a level work buffer
+    void* D;       // n FP64 field-type entries: the diagonal of D
+    void* sub;     // n FP64 field-type entries: D's subdiagonal (0 after a 1x1 pivot)
+    int* piv;      // n local indices: (P A P^T)(i, j) = A(piv[i], piv[j])
+    int64_t ld;
+    int n;
+    int k0;        // first column of the panel (a multiple of the panel width)
+    int front;     // for error reports
+};
+// Inverse of the unit lower triangle of one b x b diagonal block (ld), into
+// an NB x NB block (ld NB)
+template <class T>
+struct TriInvDesc {
+    const T* L;
+    T* Linv;
+    int64_t ld;
+    int b;
+};
+// dst(i, c) = src(i, perm[c]) for i < rows, c < cols (El::PermuteCols)
+template <class T>
+struct GatherColsDesc {
+    const T* src;
+    T* dst;
+    const int* perm;
+    int64_t lds, ldd;
+    int rows, cols;
+};
+// L(i, c) = (W D^{-1})(i, c), D block diagonal (diagonal D, subdiagonal sub)
+template <class T>
+struct QuasiDesc {
+    const T* W;
+    T* L;
+    const void* D;
+    const void* sub;
+    int64_t ld;
+    int rows, cols;
+};
+// A front's rows [off, off + n) of the nodal right-hand sides
+struct PermSeg {
+    int off, n;
+};
+
+template <class T>
+void launch_bk_panel(const BkDesc<T>* d_descs, int ndesc, int nbp, bool herm, int* d_status,
+                     hipStream_t s);
+template <class T>
+void launch_unit_lower_inverse(const TriInvDesc<T>* d_descs, int ndesc, hipStream_t s);
+// start[i] = workgroups before descriptor i (one per column and 256-row chunk)
+template <class T>
+void launch_gather_cols(const GatherColsDesc<T>* d_descs, const int* d_start, int ndesc,
+                        int nblocks, hipStream_t s);
+template <class T>
+void launch_quasi_scale(const QuasiDesc<T>* d_descs, const int* d_start, int ndesc, int nblocks,
+                        bool herm, hipStream_t s);
+// gather (scatter = false) dst(off + i) = src(off + piv[off + i]), or scatter
+// dst(off + piv[off + i]) = src(off + i), for every segment and column
+template <class T>
+void launch_permute_front_rows(const PermSeg* d_segs, const int* d_row_start, int nseg, int nrows,
+                               const T* src, T* dst, int N, int k, const int* d_piv, bool scatter,
+                               hipStream_t s);
+template <class T, class V>
+void launch_quasi_diag_solve(T* xn, const V* D, const V* sub, int N, int k, bool herm,
+                             int* d_status, hipStream_t s);
+
 // ---- launchers (stream-ordered, asynchronous) ----------------------------------
 // Clears every segment; chunk_start[i] = number of 1024-element chunks before
 // segment i.

@@ -144,7 +144,7 @@ struct IntervalAlloc {
 };
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
-                S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER };
+                S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER, S_BK, S_TRINV, S_GCOLS, S_QSCALE, S_PERM };
 
 struct Step {
     int kind;
@@ -172,6 +172,11 @@ const char* step_name(int kind) {
         case 8: return "copy";
         case 11: return "solve_reduce";
         case 12: return "zero_update";
+        case 13: return "bk_panel";
+        case 14: return "unit_lower_inverse";
+        case 15: return "permute_cols";
+        case 16: return "quasi_scale";
+        case 17: return "solve_permute";
         default: return "other";
     }
 }
@@ -376,6 +381,12 @@ struct Numeric : NumericBase {
     // block LDL work buffers, reused level by level: the fronts' original
     // A21, the unit-lower inverses of L11 and the n x 64 product panels
     DevBuf bsave, bsinv, btmp;
+    // pivoted factorization (LDL_INTRAPIV): D's subdiagonal and each front's
+    // pivot permutation, in nodal order, and its schedule
+    DevBuf subpool, pivpool;
+    Arena parena;
+    std::vector<Step> psteps;
+    bool pivBuilt = false;
 
     // A captured launch sequence (HIP graph), replayed instead of the
     // individual launches.
@@ -402,7 +413,7 @@ struct Numeric : NumericBase {
         int64_t wsUsed = 0;      // split-K workspace entries the schedule uses
         Graph graph;   // forward, diagonal and backward sweeps
     };
-    Graph fgraph[2];   // LDL and block-LDL factorizations
+    Graph fgraph[3];   // LDL, block-LDL and pivoted LDL factorizations
     std::map<int, std::unique_ptr<SolveSched>> solves;
     // schedules of the allocation-free solve (descriptors into the scratch array)
     std::map<int, std::unique_ptr<SolveSched>> noallocSolves;
@@ -413,6 +424,8 @@ struct Numeric : NumericBase {
         fpool.alloc(std::max<int64_t>(P.factorEntries, 1) * sizeof(T));
         upool.alloc(std::max<int64_t>(P.updateEntries, 1) * sizeof(T));
         dpool.alloc(std::max(P.N, 1) * sizeof(V));
+        subpool.alloc(std::max(P.N, 1) * sizeof(V));
+        pivpool.alloc(std::max(P.N, 1) * sizeof(int));
         linv.alloc(std::max<int64_t>(P.linvEntries, 1) * sizeof(T));
         linvw.alloc(std::max<int64_t>(P.linvWEntries, 1) * sizeof(T));
         {
@@ -437,11 +450,13 @@ struct Numeric : NumericBase {
     }
 
     bool block() const { return P.frontType == 1; }
+    bool pivoted() const { return P.frontType == 2; }
 
     size_t device_bytes() const override {
         size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
                    barena.dev.bytes + bsave.bytes + bsinv.bytes + btmp.bytes + rbuf.bytes +
-                   identity.bytes + linvw.bytes + wtmp.bytes + wbuf.bytes;
+                   identity.bytes + linvw.bytes + wtmp.bytes + wbuf.bytes + subpool.bytes +
+                   pivpool.bytes + parena.dev.bytes;
         for (auto& kv : solves)
             b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
                  kv.second->xn2.bytes + kv.second->tmp.bytes + kv.second->splitWs.bytes;
@@ -451,6 +466,8 @@ struct Numeric : NumericBase {
     T* L(int s) const { return fpool.as<T>() + P.fronts[s].L; }
     T* U(int s) const { return upool.as<T>() + P.fronts[s].U; }
     V* D(int s) const { return dpool.as<V>() + P.fronts[s].off; }
+    V* Sub(int s) const { return subpool.as<V>() + P.fronts[s].off; }
+    int* Piv(int s) const { return pivpool.as<int>() + P.fronts[s].off; }
     T* Li(int s, int k0) const { return linv.as<T>() + P.fronts[s].Linv + (int64_t)(k0 / NB) * NB * NB; }
     // wide inverse block starting at column w0 (a multiple of SB), ld min(SB, n - w0)
     T* LW(int s, int w0) const {
@@ -464,7 +481,8 @@ struct Numeric : NumericBase {
     // the last (X21_j = -(X22 L21_j) Linv_jj, X_jj = Linv_jj) in a separate
     // n x n buffer; ATL := X^T D^{-1} X (X^H for Hermitian) written to both
     // triangles; and A21 restored.
-    void build_factor_schedule(Arena& arena, std::vector<Step>& steps, bool blockLdl) {
+    void build_factor_schedule(Arena& arena, std::vector<Step>& steps, bool blockLdl,
+                               bool pivot = false) {
         StepBuilder<T> sb{arena, steps};
         const int herm = P.conjugate ? G_CONJB : 0;
         std::vector<int64_t> saveOff(P.numFronts, 0), sinvOff(P.numFronts, 0),
@@ -572,7 +590,8 @@ struct Numeric : NumericBase {
             // inside an OB-wide outer panel, then one rank-OB update of the
             // front's trailing columns (K = OB instead of K = 64 passes).
             const int OB = outer_panel();
-            for (int k0 = 0; k0 < maxn; k0 += OB) {
+            if (pivot) pivoted_level(sb, arena, steps, lv, Wf);
+            for (int k0 = 0; k0 < (pivot ? 0 : maxn); k0 += OB) {
                 for (int k = k0; k < std::min(maxn, k0 + OB); k += NB) {
                     std::vector<DiagDesc<T>> dd;
                     std::vector<GemmDesc<T>> trsm, upd;
@@ -670,6 +689,125 @@ struct Numeric : NumericBase {
             else wide_inverse(sb, lv);
         }
         arena.finish();
+    }
+
+    // Panel width of the pivoted LDL^T of ATL (CLIQG_BK_PANEL, 8..64): the
+    // panel's delayed updates are a matrix-vector product per column inside
+    // one workgroup, the trailing updates MFMA launches of rank width + 1.
+    static int bk_panel() {
+        static const int v = [] {
+            const char* e = std::getenv("CLIQG_BK_PANEL");
+            const int t = e ? std::atoi(e) : 32;
+            return (t >= 8 && t <= BK_MAX_PANEL) ? t : 32;
+        }();
+        return v;
+    }
+
+    // FrontLDLIntraPiv (ldl/local_front.hpp:73-94) for the fronts of one
+    // level, between the extend-add of the children's L parts and the Schur
+    // update: the pivoted LDL^T of ATL panel by panel (k_bk_panel, then the
+    // rank nbp + 1 update of ATL's trailing lower triangle), the inverses of
+    // L11's 64-wide diagonal blocks, ABL's columns permuted into the W buffer
+    // (El::PermuteCols), W21 := ABL L11^{-T} (^{-H} if Hermitian) by 64-wide
+    // blocks inside OB-wide outer panels as in the unpivoted path, and L21 :=
+    // W21 D^{-1} with 2x2 pivots (El::QuasiDiagonalSolve). The Schur update
+    // ABR -= W21 L21^T that follows is the unpivoted path's.
+    template <class FW>
+    void pivoted_level(StepBuilder<T>& sb, Arena& arena, std::vector<Step>& steps,
+                       const std::vector<int>& lv, FW Wf) {
+        const int herm = P.conjugate ? G_CONJB : 0;
+        const int nbp = bk_panel();
+        int maxn = 0;
+        for (int s : lv) maxn = std::max(maxn, P.fronts[s].n);
+        for (int k0 = 0; k0 < maxn; k0 += nbp) {
+            std::vector<BkDesc<T>> bd;
+            std::vector<GemmDesc<T>> trail;
+            double by = 0.0;
+            for (int s : lv) {
+                const FrontLayout& f = P.fronts[s];
+                if (f.n <= k0) continue;
+                bd.push_back(BkDesc<T>{L(s), Wf(s), D(s), Sub(s), Piv(s), f.m, f.n, k0, s});
+                by += 3.0 * (double)(f.n - k0) * nbp * sizeof(T);
+                const int e = k0 + nbp;
+                if (e + 1 < f.n)
+                    trail.push_back(GemmDesc<T>{Wf(s) + (e + 1) + (int64_t)k0 * f.m,
+                                                L(s) + (e + 1) + (int64_t)k0 * f.m,
+                                                L(s) + (e + 1) + (int64_t)(e + 1) * f.m, nullptr,
+                                                nullptr, f.m, f.m, f.m, f.n - e - 1, f.n - e - 1,
+                                                e - k0 + 1,
+                                                G_TRANSB | G_LOWER | G_ACCUM | G_NEG | herm});
+            }
+            if (bd.empty()) continue;
+            Step st{S_BK, arena.add(bd), 0, (int)bd.size(), 0, false, 0.0, by};
+            st.variant = nbp;
+            steps.push_back(st);
+            sb.update(trail);
+        }
+        std::vector<TriInvDesc<T>> ti;
+        std::vector<GatherColsDesc<T>> gc;
+        std::vector<QuasiDesc<T>> qd;
+        std::vector<int> gstart, qstart;
+        int gblocks = 0;
+        double gby = 0.0;
+        for (int s : lv) {
+            const FrontLayout& f = P.fronts[s];
+            for (int k = 0; k < f.n; k += NB)
+                ti.push_back(TriInvDesc<T>{L(s) + k + (int64_t)k * f.m, Li(s, k), f.m,
+                                           std::min(NB, f.n - k)});
+            if (f.u == 0 || f.n == 0) continue;
+            gc.push_back(GatherColsDesc<T>{L(s) + f.n, Wf(s) + f.n, Piv(s), f.m, f.m, f.u, f.n});
+            qd.push_back(QuasiDesc<T>{Wf(s) + f.n, L(s) + f.n, D(s), Sub(s), f.m, f.u, f.n});
+            gstart.push_back(gblocks);
+            gblocks += f.n * ((f.u + 255) / 256);
+            gby += 2.0 * (double)f.u * f.n * sizeof(T);
+        }
+        if (!ti.empty())
+            steps.push_back(Step{S_TRINV, arena.add(ti), 0, (int)ti.size(), 0, false, 0.0,
+                                 2.0 * ti.size() * NB * NB * sizeof(T)});
+        if (gc.empty()) return;
+        const size_t gstartOff = arena.add(gstart);
+        steps.push_back(Step{S_GCOLS, arena.add(gc), gstartOff, (int)gc.size(), gblocks, false, 0.0,
+                             gby});
+        // W21 := W21 L11^{-T}, 64-wide blocks inside OB-wide outer panels
+        const int OB = outer_panel();
+        for (int k0 = 0; k0 < maxn; k0 += OB) {
+            for (int k = k0; k < std::min(maxn, k0 + OB); k += NB) {
+                std::vector<GemmDesc<T>> trsm, upd;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    if (f.n <= k || f.u == 0) continue;
+                    const int b = std::min(NB, f.n - k);
+                    const int c1 = std::min(f.n, k0 + OB);
+                    T* wb = Wf(s) + f.n;
+                    trsm.push_back(GemmDesc<T>{wb + (int64_t)k * f.m, Li(s, k), wb + (int64_t)k * f.m,
+                                               nullptr, nullptr, f.m, NB, f.m, f.u, b, b,
+                                               G_TRANSB | herm});
+                    if (c1 - k - b > 0)
+                        upd.push_back(GemmDesc<T>{wb + (int64_t)k * f.m,
+                                                  L(s) + (k + b) + (int64_t)k * f.m,
+                                                  wb + (int64_t)(k + b) * f.m, nullptr, nullptr, f.m,
+                                                  f.m, f.m, f.u, c1 - k - b, b,
+                                                  G_TRANSB | G_ACCUM | G_NEG | herm});
+                }
+                sb.gemm(trsm, true);
+                sb.update(upd);
+            }
+            std::vector<GemmDesc<T>> outer;
+            for (int s : lv) {
+                const FrontLayout& f = P.fronts[s];
+                const int c1 = std::min(f.n, k0 + OB);
+                if (f.n <= k0 || f.n - c1 <= 0 || f.u == 0) continue;
+                T* wb = Wf(s) + f.n;
+                outer.push_back(GemmDesc<T>{wb + (int64_t)k0 * f.m, L(s) + c1 + (int64_t)k0 * f.m,
+                                            wb + (int64_t)c1 * f.m, nullptr, nullptr, f.m, f.m, f.m,
+                                            f.u, f.n - c1, c1 - k0,
+                                            G_TRANSB | G_ACCUM | G_NEG | herm});
+            }
+            sb.update(outer);
+        }
+        const size_t qstartOff = arena.add(gstart);
+        steps.push_back(Step{S_QSCALE, arena.add(qd), qstartOff, (int)qd.size(), gblocks, false, 0.0,
+                             gby});
     }
 
     // Inverses of the SB-wide diagonal blocks of each L11 (used by the LDL
@@ -855,7 +993,7 @@ struct Numeric : NumericBase {
 
     SolveSched& schedule(int k) {
         const bool blk = block();
-        const int key = 2 * k + (blk ? 1 : 0);
+        const int key = 4 * k + P.frontType;
         auto it = solves.find(key);
         if (it != solves.end()) return *it->second;
         auto sp = std::make_unique<SolveSched>();
@@ -885,7 +1023,7 @@ struct Numeric : NumericBase {
     // the split workspace the launches need, once with the final addresses.
     SolveSched& noalloc_schedule(int k, char* base, size_t cap) {
         const bool blk = block();
-        const int key = 2 * k + (blk ? 1 : 0);
+        const int key = 4 * k + P.frontType;
         auto it = noallocSolves.find(key);
         if (it != noallocSolves.end()) return *it->second;
         auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -1009,6 +1147,30 @@ struct Numeric : NumericBase {
                                                  (int)ds.size(), tot, false});
                     }
                 };
+                // PermuteRows of the fronts' own rows before the forward solve
+                // and InversePermuteRows after the backward solve
+                // (lower_solve/local_front.hpp:53-62, 84-93): the rows go to
+                // the second nodal buffer permuted and are copied back
+                auto permute = [&](bool scatter) {
+                    std::vector<PermSeg> ps;
+                    std::vector<int> st;
+                    std::vector<CopyDesc<T>> back;
+                    int tot = 0;
+                    for (int s : lv) {
+                        const FrontLayout& f = P.fronts[s];
+                        if (f.n == 0) continue;
+                        ps.push_back(PermSeg{f.off, f.n});
+                        st.push_back(tot);
+                        tot += f.n;
+                        back.push_back(CopyDesc<T>{X2(s), Xs(s), N, N, f.n, k});
+                    }
+                    if (ps.empty()) return;
+                    Step x{S_PERM, S.arena.add(ps), S.arena.add(st), (int)ps.size(), tot, bwd};
+                    x.variant = scatter ? 1 : 0;
+                    x.bytes = 2.0 * tot * k * sizeof(T);
+                    steps.push_back(x);
+                    sb.copy(back);
+                };
                 if (!bwd) {
                     // W_s of fronts without columns gets no update-row product
                     std::vector<ZeroSeg<T>> zs;
@@ -1025,6 +1187,7 @@ struct Numeric : NumericBase {
                         steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
                                              (int)zs.size(), chunks, false});
                     sextend(false);
+                    if (pivoted()) permute(false);
                     if (blk) {
                         // FrontBlockLowerForwardSolve (lower_solve/local_front_block.hpp:28-48):
                         // X_s := inv(ATL) X_s through a copy of X_s
@@ -1128,6 +1291,7 @@ struct Numeric : NumericBase {
                         sb.gemm(tr, true);
                         sb.gemm(up);
                     }
+                    if (pivoted()) permute(true);
                 }
             }
         }
@@ -1187,6 +1351,28 @@ struct Numeric : NumericBase {
                 case S_SEXTEND:
                     launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
                                            s.nwork, S->wp, S->x1, P.N, P.dRel.as<int>(), k, st);
+                    break;
+                case S_BK:
+                    launch_bk_panel<T>(a.at<BkDesc<T>>(s.desc), s.ndesc, s.variant, P.conjugate,
+                                       P.dStatus.as<int>(), st);
+                    break;
+                case S_TRINV:
+                    launch_unit_lower_inverse<T>(a.at<TriInvDesc<T>>(s.desc), s.ndesc, st);
+                    break;
+                case S_GCOLS:
+                    launch_gather_cols<T>(a.at<GatherColsDesc<T>>(s.desc), a.at<int>(s.start),
+                                          s.ndesc, s.nwork, st);
+                    break;
+                case S_QSCALE:
+                    launch_quasi_scale<T>(a.at<QuasiDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                          s.nwork, P.conjugate, st);
+                    break;
+                case S_PERM:
+                    // gather (variant 0) or scatter (1) of the fronts' rows from
+                    // the first nodal buffer into the second
+                    launch_permute_front_rows<T>(a.at<PermSeg>(s.desc), a.at<int>(s.start), s.ndesc,
+                                                 s.nwork, S->x1, S->x2, P.N, k, pivpool.as<int>(),
+                                                 s.variant != 0, st);
                     break;
             }
             if (prof) {
@@ -1257,6 +1443,11 @@ struct Numeric : NumericBase {
             P.ensure_dep_events(nextEvent);
             blockBuilt = true;
         }
+        if (pivoted() && !pivBuilt) {
+            build_factor_schedule(parena, psteps, false, true);
+            P.ensure_dep_events(nextEvent);
+            pivBuilt = true;
+        }
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
         // host-staged fronts replace their assembled values: converted to
         // the working type on the host, copied in after the assembly
@@ -1275,10 +1466,12 @@ struct Numeric : NumericBase {
             }
             if (block())
                 run(bsteps, barena, nullptr, 0);
+            else if (pivoted())
+                run(psteps, parena, nullptr, 0);
             else
                 run(fsteps, farena, nullptr, 0);
         };
-        if (!P.stagedFronts.empty() || !replay(fgraph[block() ? 1 : 0], body)) body();
+        if (!P.stagedFronts.empty() || !replay(fgraph[P.frontType], body)) body();
         // the copies read the host buffers until they complete
         CQ_HIP(hipStreamSynchronize(st));
         P.stagedFronts.clear();
@@ -1334,7 +1527,10 @@ struct Numeric : NumericBase {
             run(S.fwd, S.arena, &S, k);
             // block fronts hold inv(ATL), so there is no separate diagonal
             // solve (solve.hpp:48-53)
-            if (!block())
+            if (pivoted())
+                launch_quasi_diag_solve<T, V>(S.x2, dpool.as<V>(), subpool.as<V>(), P.N, k,
+                                              P.conjugate, S.status, P.stream);
+            else if (!block())
                 launch_diag_solve<T, V>(S.x2, dpool.as<V>(), P.N, k, S.status, P.stream);
             run(S.bwd, S.arena, &S, k);
         };
@@ -1347,6 +1543,16 @@ struct Numeric : NumericBase {
             rbufCols = k;
         }
         return rbuf.as<V>();
+    }
+
+    // D's subdiagonal and the pivot permutation of a front of a pivoted
+    // factorization (SymmFront::subdiag, SymmFront::piv)
+    void get_pivots(int s, void* hsub, int* hpiv) {
+        const FrontLayout& f = P.fronts[s];
+        if (f.n == 0) return;
+        CQ_HIP(hipStreamSynchronize(P.stream));
+        if (hsub) CQ_HIP(hipMemcpy(hsub, Sub(s), f.n * sizeof(V), hipMemcpyDeviceToHost));
+        if (hpiv) CQ_HIP(hipMemcpy(hpiv, Piv(s), f.n * sizeof(int), hipMemcpyDeviceToHost));
     }
 
     void get_front(int s, void* hL, void* hdiag) {
@@ -1591,7 +1797,7 @@ static void dispatch(bool complex, int precision, F&& f) {
 void Plan::factor(const void* d_vals, int prec, int ftype) {
     after_caller();
     if (prec != 0 && prec != 1) throw Error(1, "unknown precision");
-    if (ftype != 0 && ftype != 1) throw Error(1, "unknown front type");
+    if (ftype != 0 && ftype != 1 && ftype != 2) throw Error(1, "unknown front type");
     if (!num || prec != precision) {
         num.reset();
         precision = prec;
@@ -1785,6 +1991,16 @@ void Plan::set_front(int s, const void* hL) {
     // is kept zero, as the assembly leaves it
     const int m = fronts[s].m, n = fronts[s].n;
     for (int j = 1; j < n; ++j) std::memset(buf.data() + (size_t)j * m * vb, 0, (size_t)j * vb);
+}
+
+void Plan::get_pivots(int s, void* hsub, int* hpiv) {
+    if (!factored) throw Error(6, "the plan has not been factored");
+    if (frontType != 2) throw Error(6, "the factorization is not pivoted");
+    if (s < 0 || s >= numFronts) throw Error(1, "front index out of range");
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        static_cast<Numeric<T>*>(num.get())->get_pivots(s, hsub, hpiv);
+    });
 }
 
 void Plan::get_front(int s, void* hL, void* hdiag) {

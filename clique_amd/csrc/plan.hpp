@@ -71,6 +71,9 @@ public:
     void adjoint(const void* d_g, int64_t ldg, const void* d_x, int64_t ldx, int k,
                  void* d_lambda, int64_t ldl, void* d_dA);
     void get_front(int s, void* h_L, void* h_diag);
+    // pivoted factorizations: D's subdiagonal (n FP64 field values) and the
+    // pivot permutation (n local indices) of front s
+    void get_pivots(int s, void* h_sub, int* h_piv);
     // stage front s's unfactored values (m x n, FP64 field type) for the
     // next factorization, which uses them in place of A's assembled values
     void set_front(int s, const void* h_L);
@@ -99,7 +102,7 @@ public:
 
     DevBuf dRowOff, dCols, dAsmDst, dRel, dLow, dMap, dVals, dStatus, dNorms;
     std::vector<int> relHost;   // relative indices of every front (host copy of dRel)
-    int precision = -1, frontType = 0;
+    int precision = -1, frontType = 0;   // frontType: cliqg_front_type
     bool factored = false;
     double normA = 0.0;
 

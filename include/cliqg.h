@@ -51,8 +51,12 @@ typedef enum { CLIQG_F64 = 0, CLIQG_F32 = 1 } cliqg_precision;
  * (include/clique/numeric/dist_symm_front_tree/decl.hpp:17-47). 1D/2D
  * distinctions are layout choices for P>1 and collapse at world size 1. */
 typedef enum {
-    CLIQG_FRONT_LDL = 0,        /* LDL_1D / LDL_2D: unit L, diagonal D          */
-    CLIQG_FRONT_BLOCK_LDL = 1   /* BLOCK_LDL_*: ATL := inverse, ABL original A21 */
+    CLIQG_FRONT_LDL = 0,          /* LDL_1D / LDL_2D: unit L, diagonal D            */
+    CLIQG_FRONT_BLOCK_LDL = 1,    /* BLOCK_LDL_*: ATL := inverse, ABL original A21   */
+    CLIQG_FRONT_LDL_INTRAPIV = 2  /* LDL_INTRAPIV_*: Bunch-Kaufman pivoting inside
+                                     each front's ATL (FrontLDLIntraPiv,
+                                     ldl/local_front.hpp:73-94): unit L, D with
+                                     1x1 and 2x2 blocks, a permutation per front  */
 } cliqg_front_type;
 
 /* which-array selectors for cliqg_symbolic_node_array */
@@ -211,6 +215,15 @@ cliqg_status cliqg_plan_adjoint(cliqg_plan plan, const void* d_g, int ldg, const
  * precision), h_diag the n pivots (LDL fronts only; may be NULL). Replaces
  * reading frontTree.localFronts[s].frontL/diag (tests/Refactor.cpp:157-160). */
 cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_diag);
+
+/* Pivots of front s of a CLIQG_FRONT_LDL_INTRAPIV factorization: h_subdiag
+ * gets D's n subdiagonal entries (FP64 or complex FP64; entry i is D(i+1, i),
+ * nonzero exactly where a 2x2 pivot starts at i), h_piv the n local indices
+ * of the front's permutation, (P ATL P^T)(i, j) = ATL(piv[i], piv[j]).
+ * h_diag of cliqg_plan_get_front gets D's diagonal. Replaces reading
+ * localFronts[s].subdiag and .piv (dist_symm_front_tree/decl.hpp:174-185,
+ * filled by ldl/local.hpp:108-115). Either pointer may be NULL. */
+cliqg_status cliqg_plan_get_front_pivots(cliqg_plan plan, int s, void* h_subdiag, int* h_piv);
 
 /* Stage front s's unfactored values for the next cliqg_plan_factor, which
  * factors them in place of A's assembled values for that front (the other

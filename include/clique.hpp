@@ -821,12 +821,24 @@ public:
                                                    ? nullptr
                                                    : localFronts[s].diag.Buffer()),
                           "MirrorFronts");
+            // pivoted fronts also keep D's subdiagonal and the permutation
+            // (ldl/local.hpp:108-115)
+            if (PivotedFactorization(frontType)) {
+                localFronts[s].subdiag.Resize(n, 1);
+                localFronts[s].piv.Resize(n, 1);
+                detail::check(cliqg_plan_get_front_pivots(plan_.get(), s,
+                                                          localFronts[s].subdiag.Buffer(),
+                                                          localFronts[s].piv.Buffer()),
+                              "MirrorFronts");
+            }
         }
         distFronts.assign(1, DistSymmFront<F>());
         if (num) {
             distFronts[0].front1dL = localFronts.back().frontL;
             distFronts[0].front2dL = localFronts.back().frontL;
             distFronts[0].diag = localFronts.back().diag;
+            distFronts[0].subdiag = localFronts.back().subdiag;
+            distFronts[0].piv = localFronts.back().piv;
         }
     }
 
@@ -930,11 +942,12 @@ inline void LDL(DistSymmInfo& info, DistSymmFrontTree<F>& L, SymmFrontType newTy
     if (!Unfactored(L.frontType)) throw LogicError("Matrix is already factored");
     ChangeFrontType(L, SYMM_2D);
     const SymmFrontType initial = InitialFactorType(newType);
-    if (PivotedFactorization(initial))
-        throw LogicError("Intra-front pivoting (Bunch-Kaufman) is not available in this build");
+    if (BlockFactorization(initial) && PivotedFactorization(initial))
+        throw LogicError("Block LDL with intra-front pivoting is not available in this build");
     if (L.hostFronts_) L.PushFronts();
-    const cliqg_front_type ft = BlockFactorization(initial) ? CLIQG_FRONT_BLOCK_LDL
-                                                            : CLIQG_FRONT_LDL;
+    const cliqg_front_type ft = BlockFactorization(initial)     ? CLIQG_FRONT_BLOCK_LDL
+                                : PivotedFactorization(initial) ? CLIQG_FRONT_LDL_INTRAPIV
+                                                                : CLIQG_FRONT_LDL;
     detail::check(cliqg_plan_factor(L.Plan(), L.DeviceValues(),
                                     precision == FACTOR_FP32 ? CLIQG_F32 : CLIQG_F64, ft),
                   "LDL");
@@ -1092,12 +1105,13 @@ inline void Multiply(T alpha, const DistSparseMatrix<T>& A, const DistMultiVec<T
 }
 
 // Black-box solve (numeric/solve.hpp:94-116) on a grid graph: natural
-// dissection of the nx x ny x nz grid, LDL and Solve. The reference's
+// dissection of the nx x ny x nz grid, LDL with intra-front pivoting
+// (LDL_INTRAPIV_1D, the reference's choice at numeric/solve.hpp:110) and Solve. The reference's
 // METIS-based overload without grid dimensions is not available.
 template <typename F>
 inline void SymmetricSolve(const DistSparseMatrix<F>& A, DistMultiVec<F>& X, int nx, int ny,
                            int nz, bool conjugate = false, int cutoff = 128,
-                           SymmFrontType frontType = LDL_1D) {
+                           SymmFrontType frontType = LDL_INTRAPIV_1D) {
     DistSymmInfo info;
     DistSeparatorTree sepTree;
     DistMap map, inverseMap;

@@ -2,7 +2,7 @@
 // operator, random X, Y := A X, natural nested dissection, LDL, Pull, Solve,
 // Push, error norms), written against the cliq:: surface of clique.hpp.
 // Exits non-zero if the relative error or the backward error is too large.
-//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [real|complex] [ldl|block] [1d|2d]
+//   solve_driver n1 n2 n3 numRhs [fp64|fp32] [real|complex] [ldl|block|intrapiv] [1d|2d]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -108,6 +108,22 @@ int run(int n1, int n2, int n3, int numRhs, bool fp32, SymmFrontType ftype) {
         YNodal.Push(inverseMap, info, Xc);
         DistNodalMatrix<F> M(inverseMap, info, Y);
         try { Solve(info, frontTree, M); } catch (const LogicError&) { wrongKindThrew = true; }
+        if (PivotedFactorization(ftype)) {
+            // the mirrored pivots of every front form a permutation, and the
+            // 2x2 pivots (nonzero subdiagonal) never end a front
+            frontTree.MirrorFronts();
+            for (const SymmFront<F>& fr : frontTree.localFronts) {
+                const int n = fr.piv.Height();
+                std::vector<int> seen(n, 0);
+                for (int i = 0; i < n; ++i) {
+                    const int p = fr.piv.Get(i, 0);
+                    checks = checks && p >= 0 && p < n && !seen[p];
+                    if (p >= 0 && p < n) seen[p] = 1;
+                }
+                checks = checks && fr.subdiag.Height() == n &&
+                         (n == 0 || fr.subdiag.Get(n - 1, 0) == F(0));
+            }
+        }
         // device norms of the pushed (device-current) solution against the
         // host loop
         std::vector<typename Base<F>::type> dn;
@@ -184,8 +200,11 @@ int main(int argc, char* argv[]) {
     const bool fp32 = argc > 5 && std::string(argv[5]) == "fp32";
     const bool cpx = argc > 6 && std::string(argv[6]) == "complex";
     const bool blk = argc > 7 && std::string(argv[7]) == "block";
+    const bool piv = argc > 7 && std::string(argv[7]) == "intrapiv";
     const bool twoD = argc > 8 && std::string(argv[8]) == "2d";
-    const SymmFrontType ftype = blk ? (twoD ? BLOCK_LDL_2D : BLOCK_LDL_1D) : (twoD ? LDL_2D : LDL_1D);
+    const SymmFrontType ftype = blk   ? (twoD ? BLOCK_LDL_2D : BLOCK_LDL_1D)
+                                : piv ? (twoD ? LDL_INTRAPIV_2D : LDL_INTRAPIV_1D)
+                                      : (twoD ? LDL_2D : LDL_1D);
     int rc = 1;
     try {
         rc = cpx ? run<std::complex<double>>(n1, n2, n3, k, fp32, ftype)

@@ -17,7 +17,10 @@ DRIVER = os.path.join(ROOT, "tests", "cpp", "solve_driver")
                                   ["16", "14", "12", "3", "fp64", "real", "block"],
                                   ["12", "12", "12", "2", "fp32", "complex", "block"],
                                   ["14", "13", "12", "2", "fp64", "complex", "ldl", "2d"],
-                                  ["12", "11", "10", "2", "fp64", "real", "block", "2d"]])
+                                  ["12", "11", "10", "2", "fp64", "real", "block", "2d"],
+                                  ["13", "12", "11", "3", "fp64", "complex", "intrapiv"],
+                                  ["14", "12", "10", "2", "fp32", "real", "intrapiv"],
+                                  ["11", "12", "13", "2", "fp64", "real", "intrapiv", "2d"]])
 def test_cpp_driver(device, args):
     if not os.path.exists(DRIVER):
         pytest.fail("tests/cpp/solve_driver is not built (run __graft_entry__.build())")

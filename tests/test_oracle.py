@@ -15,7 +15,7 @@ import scipy.sparse.linalg as spla
 
 import oracle
 from clique_amd import problems as P
-from helpers import backward_error, hermitian_matrix, problem, rel_diff
+from helpers import backward_error, hermitian_matrix, indefinite_box, problem, rel_diff
 
 
 def tree_of(kind, n, cut, **kw):
@@ -341,27 +341,6 @@ def test_blas_baseline_matches_oracle(kind, n, cut, cplx, conj):
     xa = ref.solve_nodal(b)
     xb = f.solve_nodal_blas(b)
     assert rel_diff(xb, xa) <= 1e-12
-
-
-def indefinite_box(dims, seed=0, cplx=False, herm=False):
-    """A 7-point operator with a weak diagonal of random sign: indefinite, so
-    Bunch-Kaufman takes swaps and 2x2 pivots."""
-    ip, ind, v, dims = P.stencil7(*dims, 1.0, -1.0)
-    rng = np.random.default_rng(seed)
-    N = ip.size - 1
-    rows = np.repeat(np.arange(N), np.diff(ip))
-    v = v.astype(np.complex128) if cplx else v.copy()
-    diag = rows == ind
-    v[diag] = rng.uniform(-0.6, 0.6, N) + (1j * rng.uniform(-0.3, 0.3, N) if cplx and not herm else 0)
-    if herm:
-        # complex couplings, Hermitian: (j,i) holds the conjugate of (i,j)
-        ph = np.exp(1j * rng.uniform(0, 2 * np.pi, ind.size))
-        A = sp.csr_matrix((v * np.where(diag, 1, ph), ind, ip), shape=(N, N))
-        A = sp.triu(A, 1) + sp.triu(A, 1).conj().T + sp.diags(A.diagonal().real)
-        A = A.tocsr()
-        A.sort_indices()
-        return A.indptr.astype(np.int64), A.indices.astype(np.int32), A.data, dims
-    return ip, ind, v, dims
 
 
 def _dense_nodal(ip, ind, v, t):
