@@ -163,15 +163,18 @@ def cpu_baseline(budget_s=20.0, k=64, target=None):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_gemm_update_pml120_fp32_traffic.json")
 
 
-def measured_traffic(cfg, n):
+def measured_traffic(cfg, n, family):
     """HBM bytes per launch of the factor's update GEMM, from the committed
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE over `bench.py
     --factor-only`, each scaled by the same counter on a copy of known size);
-    None for other workloads or when no measurement is committed."""
+    None for other workloads, for another kernel than the one measured, or
+    when no measurement is committed."""
     if cfg != 5 or n not in (None, 120) or not os.path.exists(TRAFFIC_FILE):
         return None, None
     with open(TRAFFIC_FILE) as fh:
         t = json.load(fh)
+    if t.get("family", "gemm_update") != family:
+        return None, None
     return t["bytes_per_launch"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
@@ -186,6 +189,8 @@ def factor_only(args):
     sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
     vals = torch.tensor(v, device="cuda:0")
     p = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+    if not args.no_tune:
+        p.autotune(vals, precision="f32")
     p.set_profiling(True)
     for _ in range(2):
         p.factor(vals, precision="f32")
@@ -194,11 +199,13 @@ def factor_only(args):
     dst = torch.empty_like(src)
     dst.copy_(src)
     torch.cuda.synchronize()
-    g = prof["gemm_update"]
+    name = max((n_ for n_ in prof if n_.startswith("gemm_update")), key=lambda n_: prof[n_]["ms"])
+    g = prof[name]
     print(json.dumps({"factor_ms": p.last_factor_ms(), "copy_bytes": src.numel() * 4,
-                      "gemm_update_launches": g["launches"],
-                      "gemm_update_avg_ms": g["ms"] / g["launches"],
-                      "gemm_update_tflops": g["flops"] / (g["ms"] * 1e-3) / 1e12}))
+                      "family": name, "launches": g["launches"],
+                      "avg_ms": g["ms"] / g["launches"],
+                      "tflops": g["flops"] / (g["ms"] * 1e-3) / 1e12,
+                      "shapes": {kk: e["shape"] for kk, e in cq.autotune_table().items()}}))
 
 
 def timed_solve(p, src, reps=3):
@@ -243,6 +250,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the GEMM tile autotuning pass (cost-model tile shapes)")
     ap.add_argument("--block", action="store_true",
                     help="add a BLOCK_LDL (selective inversion) FP32+IR line")
     ap.add_argument("--no-block", action="store_true", help=argparse.SUPPRESS)
@@ -283,6 +292,12 @@ def main():
           f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
     p32 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+    tune = {}
+    if not args.no_tune:
+        # the project's tile autotuner, once per plan before the timed steps
+        t_tune = time.perf_counter()
+        nb, nc = p32.autotune(vals, precision="f32", rhs=k)
+        tune["fp32"] = {"buckets": nb, "changed": nc, "s": time.perf_counter() - t_tune}
 
     def step32():
         p32.factor(vals, precision="f32")
@@ -365,6 +380,10 @@ def main():
     if not args.no_fp64:
         p64 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
         x64 = cq.colmajor(torch.empty_like(bd))
+        if not args.no_tune:
+            t_tune = time.perf_counter()
+            nb, nc = p64.autotune(vals, precision="f64", rhs=k)
+            tune["fp64"] = {"buckets": nb, "changed": nc, "s": time.perf_counter() - t_tune}
 
         def step64():
             p64.factor(vals, precision="f64")
@@ -397,18 +416,25 @@ def main():
                 "factor_tflops": flops / (np.mean(f64_ms) * 1e-3) / 1e12,
                 "refine_iterations": infos64[-1]["iterations"],
                 "backward_error": infos64[-1]["backward_error"],
-                "gemm_update_tflops": prof64["gemm_update"]["flops"]
-                / (prof64["gemm_update"]["ms"] * 1e-3) / 1e12}
+                "gemm_update_tflops": sum(e["flops"] for n_, e in prof64.items()
+                                          if n_.startswith("gemm_update"))
+                / (sum(e["ms"] for n_, e in prof64.items() if n_.startswith("gemm_update")) * 1e-3)
+                / 1e12}
         print(f"# fp64: {t64 * 1e3:.1f} ms/step, factor {np.mean(f64_ms):.1f} ms",
               file=sys.stderr, flush=True)
         del p64
 
-    g = prof32["gemm_update"]
+    # the dominant kernel: the update family (one tile shape) with the most time
+    upd_name = max((n_ for n_ in prof32 if n_.startswith("gemm_update")),
+                   key=lambda n_: prof32[n_]["ms"])
+    g = prof32[upd_name]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
+    kernel_of = {"gemm_update": "k_gemm<cfloat,64,64,3>", "gemm_update_w8": "k_gemm<cfloat,128,128,3,16,2,4>",
+                 "gemm_update_small": "k_gemm<cfloat,32,32,3>", "gemm_update_large": "k_gemm<float,128,128,3>"}
     # complex FP32 products issue three real MFMAs per complex block (Gauss)
     # where the flop convention counts four real multiplications
     mfma_share = 0.75 if cplx else 1.0
-    traffic, traffic_src = measured_traffic(args.config, args.n)
+    traffic, traffic_src = measured_traffic(args.config, args.n, upd_name)
     solve_names = ("solve_gemm", "solve_extend_add", "solve_gather", "solve_zero", "solve_reduce")
 
     def families(prof, solve):
@@ -445,7 +471,9 @@ def main():
         "fp64": fp64,
         "speedup_fp32ir_vs_fp64": (fp64["time_to_tol_ms"] / (t32 * 1e3)) if fp64 else None,
         "roofline": {"bound": "mfma",
-                     "kernel": "k_gemm<cfloat,64,64,3> (Schur, panel and outer-panel updates, FP32 factor)",
+                     "kernel": kernel_of.get(upd_name, upd_name) +
+                     " (Schur, panel and outer-panel updates, FP32 factor)",
+                     "family": upd_name,
                      "achieved": achieved, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["fp32"],
                      "mfma_issue_frac": achieved * mfma_share / PEAK["fp32"],
@@ -456,6 +484,8 @@ def main():
         "breakdown_fp32_factor": breakdown,
         "breakdown_fp32_solves": families(prof32, True),
         "block_ldl_fp32_ir": blk,
+        "autotune": tune or None,
+        "autotune_shapes": {kk: e["shape"] for kk, e in cq.autotune_table().items()},
     }
     if not args.no_cpu:
         import subprocess

@@ -203,6 +203,26 @@ class DistSymmFrontTree:
         self._vals = vals
         self.factor_count += 1
 
+    def autotune(self, vals, precision="f64", front_type="ldl", rhs=0):
+        """Time every admissible GEMM tile shape of this factorization (and of
+        a solve with `rhs` right-hand sides), keep the fastest per launch
+        bucket in the process-wide table, and factor `vals` again with the
+        tuned shapes. Returns (buckets measured, buckets changed)."""
+        import torch
+        want = torch.complex128 if self.is_complex else torch.float64
+        if vals.dtype != want or not vals.is_cuda or vals.numel() != self.nnz:
+            raise TypeError(f"values must be a CUDA {want} tensor of {self.nnz} entries")
+        vals = vals.contiguous()
+        nb, nc = ctypes.c_int(), ctypes.c_int()
+        _lib.check(lib().cliqg_plan_autotune(self._h, _dev_ptr(vals), _PRECISIONS[precision],
+                                             _FRONT_TYPES[front_type], int(rhs), ctypes.byref(nb),
+                                             ctypes.byref(nc)))
+        self.precision = precision
+        self.front_type = front_type
+        self._vals = vals
+        self.factor_count += 1
+        return nb.value, nc.value
+
     def solve(self, x, nodal=False):
         t, ld, k = _check_rhs(x, self.N, self.is_complex)
         fn = lib().cliqg_plan_solve_nodal if nodal else lib().cliqg_plan_solve
@@ -310,6 +330,25 @@ class DistSymmFrontTree:
         v = ctypes.c_double()
         _lib.check(lib().cliqg_plan_last_factor_ms(self._h, ctypes.byref(v)))
         return v.value
+
+
+_SHAPES = {0: "64x64", 4: "128x128", 5: "32x32", 6: "128x128x8w"}
+
+
+def autotune_table():
+    """The GEMM tile autotuner's table: {bucket: {"shape": ..., "ms": {shape: ms}}}."""
+    n = ctypes.c_size_t()
+    _lib.check(lib().cliqg_autotune_table(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _lib.check(lib().cliqg_autotune_table(buf, n.value + 1, ctypes.byref(n)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        key, v, *ms = line.split()
+        out[key] = {"shape": _SHAPES.get(int(v), v),
+                    "ms": {name: float(m) for name, m in zip(("32x32", "64x64", "128x128",
+                                                              "128x128x8w"), ms)
+                           if float(m) >= 0}}
+    return out
 
 
 def LDL(info, L: DistSymmFrontTree, vals, front_type=LDL_2D, precision="f64"):

@@ -51,6 +51,7 @@ EXPORTS = [
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
     "cliqg_plan_set_graphs", "cliqg_plan_solve_noalloc", "cliqg_plan_set_front",
     "cliqg_permute_rows", "cliqg_col_norms", "cliqg_plan_get_front_pivots",
+    "cliqg_plan_autotune", "cliqg_autotune_table",
 ]
 
 
@@ -99,6 +100,8 @@ def lib():
         "cliqg_plan_get_front": (c_int, [vp, c_int, vp, vp]),
         "cliqg_plan_set_front": (c_int, [vp, c_int, vp]),
         "cliqg_plan_get_front_pivots": (c_int, [vp, c_int, vp, ip]),
+        "cliqg_plan_autotune": (c_int, [vp, vp, c_int, c_int, c_int, ip, ip]),
+        "cliqg_autotune_table": (c_int, [ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
         "cliqg_permute_rows": (c_int, [c_int, c_int, vp, c_int, vp, c_int, vp, c_int, c_int, vp]),
         "cliqg_col_norms": (c_int, [c_int, c_int, vp, c_int, c_int, dp, vp]),
         "cliqg_plan_last_factor_ms": (c_int, [vp, dp]),

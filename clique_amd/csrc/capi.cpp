@@ -389,6 +389,31 @@ cliqg_status cliqg_plan_get_front(cliqg_plan plan, int s, void* h_L, void* h_dia
     return guarded([&] { P(plan).get_front(s, h_L, h_diag); });
 }
 
+cliqg_status cliqg_plan_autotune(cliqg_plan plan, const void* d_vals, cliqg_precision precision,
+                                 cliqg_front_type front_type, int solve_rhs, int* h_buckets,
+                                 int* h_changed) {
+    return guarded([&] {
+        auto& p = P(plan);
+        require(d_vals || p.nnz == 0, "null values");
+        int nb = 0, nc = 0;
+        p.autotune(d_vals, (int)precision, (int)front_type, solve_rhs, &nb, &nc);
+        if (h_buckets) *h_buckets = nb;
+        if (h_changed) *h_changed = nc;
+    });
+}
+
+cliqg_status cliqg_autotune_table(char* h_buf, size_t capacity, size_t* h_length) {
+    return guarded([&] {
+        const std::string t = cliqg::tune_table_text();
+        if (h_length) *h_length = t.size();
+        if (h_buf && capacity > 0) {
+            const size_t n = std::min(capacity - 1, t.size());
+            std::memcpy(h_buf, t.data(), n);
+            h_buf[n] = 0;
+        }
+    });
+}
+
 cliqg_status cliqg_plan_get_front_pivots(cliqg_plan plan, int s, void* h_subdiag, int* h_piv) {
     return guarded([&] { P(plan).get_pivots(s, h_subdiag, h_piv); });
 }

@@ -164,7 +164,9 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 
 }  // namespace
 
-int gemm_tile_m(int variant) { return variant == TILE_32 ? 32 : (variant == TILE_128 ? 128 : 64); }
+int gemm_tile_m(int variant) {
+    return variant == TILE_32 ? 32 : ((variant == TILE_128 || variant == TILE_128_8W) ? 128 : 64);
+}
 int gemm_tile_n(int variant) { return gemm_tile_m(variant); }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
@@ -181,15 +183,16 @@ int gemm_tiles(int M, int N, int flags, int variant) {
 }
 
 // ---------------------------------------------------------------------------
-// Grouped GEMM: one TM x TN output tile per workgroup of 4 waves (2 x 2);
-// each wave owns a (TM/2) x (TN/2) quadrant of 16x16 MFMA blocks. K is
+// Grouped GEMM: one TM x TN output tile per workgroup of WM x WN waves;
+// each wave owns a (TM/WM) x (TN/WN) part of 16x16 MFMA blocks. K is
 // consumed in chunks of 16 through two LDS buffers (one barrier per chunk);
 // the next chunk is prefetched into registers while the current one feeds
-// the MFMAs. Complex products use four real MFMAs on planar (re, im) tiles.
+// the MFMAs. Complex products run on planar (re, im) tiles: three real MFMAs
+// per block for complex FP32 (below), four for complex FP64.
 // ---------------------------------------------------------------------------
 template <class T, int TM, int TN, int ROLE, int TK, int WM = 2, int WN = 2>
 __global__ __launch_bounds__(64 * WM * WN)
-__attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
+__attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 2048) ? 2
                                    : (sizeof(T) <= 8 ? 4 : 2), 8))) void k_gemm(const GemmDesc<T>* __restrict__ descs,
                                                        const int* __restrict__ tile_start,
                                                        int ndesc) {
@@ -229,7 +232,7 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 4096) ? 2
     // Complex FP32 products use three real MFMAs per block (Gauss):
     // P1 = sum aR bR, P2 = sum aI bI, P3 = sum (aR + aI)(bR + bI);
     // Re = P1 - P2, Im = P3 - P1 - P2. Complex FP64 keeps four.
-    constexpr bool M3 = CPX && !F64 && GEMM_3M && BM * BN <= 4;   // registers: small wave tiles
+    constexpr bool M3 = CPX && !F64 && GEMM_3M && BM * BN <= 8;   // registers: 64x32 wave tiles at most
     A4 accR[BM][BN], accI[BM][BN], accS[M3 ? BM : 1][M3 ? BN : 1];
 #pragma unroll
     for (int a = 0; a < BM; ++a)
@@ -513,6 +516,11 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
             launch_gemm_shape<T, 128, 128, 16, 2, 2>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
             throw Error(1, "128x128 GEMM tiles are built for real types only");
+    } else if (variant == TILE_128_8W) {
+        if constexpr (gemm_variant_allowed<T>(TILE_128_8W))
+            launch_gemm_shape<T, 128, 128, 16, 2, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+        else
+            throw Error(1, "eight-wave 128x128 GEMM tiles are built for complex FP32 only");
     } else if (variant == TILE_32) {
         launch_gemm_shape<T, 32, 32>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     } else {

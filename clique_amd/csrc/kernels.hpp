@@ -13,18 +13,26 @@ constexpr int NB = 64;
 // Tile shapes of the grouped GEMM (every launch uses one of them; the value
 // is the `variant` of a launch and appears in profile traces).
 enum GemmTile : int {
-    TILE_64 = 0,    // 64x64 tiles, four waves of 32x32 (the default)
-    TILE_128 = 4,   // 128x128 tiles, four waves of 64x64 (real types, large launches)
-    TILE_32 = 5,    // 32x32 tiles, four waves of 16x16 (latency-bound launches)
+    TILE_64 = 0,      // 64x64 tiles, four waves of 32x32 (the default)
+    TILE_128 = 4,     // 128x128 tiles, four waves of 64x64 (real types, large launches)
+    TILE_32 = 5,      // 32x32 tiles, four waves of 16x16 (latency-bound launches)
+    TILE_128_8W = 6,  // 128x128 tiles, eight waves of 64x32 (complex FP32)
 };
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
-// 128x128 tiles are built for the real types only: complex FP32 keeps its
-// three-multiplication 64x64 tiles, complex FP64 would exceed the LDS.
+// The shapes built for each working type: 128x128 tiles of four waves for
+// the real types (a complex 64x64 wave tile would not keep its
+// three-multiplication products in registers), of eight waves for complex
+// FP32 (three multiplications per block on 64x32 wave tiles); complex FP64
+// keeps 64x64 tiles (a 128x128 complex FP64 tile would fill the LDS).
 template <class T>
 constexpr bool gemm_variant_allowed(int variant) {
-    return variant != TILE_128 || !Scalar<T>::complex;
+    if (variant == TILE_128) return !Scalar<T>::complex;
+    if (variant == TILE_128_8W) return Scalar<T>::complex && sizeof(T) == 8;
+    return variant == TILE_64 || variant == TILE_32;
 }
+// Every tile shape, in the order the autotuner tries them
+constexpr int GEMM_VARIANTS[4] = {TILE_32, TILE_64, TILE_128, TILE_128_8W};
 
 enum GemmFlags : int {
     G_TRANSA = 1,   // op(A)(i,k) = A[k + i*lda]   (else A[i + k*lda])

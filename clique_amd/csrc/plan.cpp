@@ -16,6 +16,7 @@
 #include "../../include/cliqg.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -154,11 +155,136 @@ struct Step {
     double flops = 0.0;   // algorithmic flops of the launch (reference convention)
     double bytes = 0.0;   // algorithmic bytes of the launch (HBM-bound steps)
     int variant = 0;      // GEMM tile shape
+    // GEMM tile shapes the launch may take (the autotuner picks among them):
+    // shape, arena offset of its tile-start array and its tile count
+    int nvar = 0;
+    int vars[4] = {0, 0, 0, 0};
+    size_t vstart[4] = {0, 0, 0, 0};
+    int vtiles[4] = {0, 0, 0, 0};
+    int bucket = -1;      // the launch's autotuner bucket (index into the tuning table)
     bool scaled = false;  // GEMM descriptors carry pivots (scaleK)
     bool update = false;  // a factor update (profiled as gemm_update, role 3 kernels)
     int stream = 0;       // 0: the plan's stream, 1: its side stream
     int event = -1;       // S_RECORD / S_WAIT: index into the plan's dependency events
 };
+
+// ---------------------------------------------------------------------------
+// GEMM tile autotuner. Launches fall into buckets by working type, role
+// (TRSM-like product, solve product, pivot-scaled or factor update), lower
+// masking, and the base-2 logarithms of their 64x64 tile count and mean K.
+// Plan::autotune times every admissible tile shape of every grouped GEMM of
+// a factorization (and of a solve) on the device and records per bucket the
+// shape with the least total time; schedules built afterwards, and the
+// tuned plan's own, use it. The table is shared by all plans of the process
+// and, with CLIQG_TUNE_CACHE=<file>, read from and written to that file
+// (one "bucket shape" line per entry).
+// ---------------------------------------------------------------------------
+struct TuneTable {
+    std::mutex mu;
+    std::map<std::string, int> index;
+    std::vector<std::string> keys;
+    std::vector<int> choice;   // tuned shape per bucket, -1 if none
+    // milliseconds per tile shape (GEMM_VARIANTS order) summed over the
+    // bucket's launches in the last tuning pass, -1 where not measured
+    std::vector<std::array<double, 4>> ms;
+    bool loaded = false;
+};
+TuneTable& tune_table() {
+    static TuneTable t;
+    return t;
+}
+const char* tune_cache_path() {
+    static const char* p = std::getenv("CLIQG_TUNE_CACHE");
+    return p && *p ? p : nullptr;
+}
+int tune_bucket_locked(TuneTable& t, const std::string& key) {
+    auto it = t.index.find(key);
+    if (it != t.index.end()) return it->second;
+    const int b = (int)t.keys.size();
+    t.index.emplace(key, b);
+    t.keys.push_back(key);
+    t.choice.push_back(-1);
+    t.ms.push_back({-1.0, -1.0, -1.0, -1.0});
+    return b;
+}
+void tune_load_locked(TuneTable& t) {
+    if (t.loaded) return;
+    t.loaded = true;
+    const char* path = tune_cache_path();
+    if (!path) return;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return;
+    char key[128];
+    int v;
+    while (std::fscanf(f, "%127s %d", key, &v) == 2) {
+        bool known = false;
+        for (int g : GEMM_VARIANTS) known = known || g == v;
+        if (known) t.choice[tune_bucket_locked(t, key)] = v;
+    }
+    std::fclose(f);
+}
+int tune_bucket(int type, int role, bool lower, double tiles64, double meanK) {
+    auto lg = [](double x) { return x >= 1.0 ? (int)std::lround(std::log2(x)) : 0; };
+    char key[128];
+    std::snprintf(key, sizeof(key), "t%d.r%d.l%d.T%d.K%d", type, role, lower ? 1 : 0, lg(tiles64),
+                  lg(meanK));
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    tune_load_locked(t);
+    return tune_bucket_locked(t, key);
+}
+// CLIQG_GEMM_TILE=<shape>: every launch that admits the shape takes it
+// (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave), ahead of the
+// tuned and the modelled choice
+int forced_variant() {
+    static const int v = [] {
+        const char* e = std::getenv("CLIQG_GEMM_TILE");
+        if (!e || !*e) return -1;
+        const int t = std::atoi(e);
+        for (int g : GEMM_VARIANTS)
+            if (g == t) return t;
+        return -1;
+    }();
+    return v;
+}
+int tuned_variant(int bucket) {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    return (bucket >= 0 && bucket < (int)t.choice.size()) ? t.choice[bucket] : -1;
+}
+void tune_save_locked(TuneTable& t) {
+    const char* path = tune_cache_path();
+    if (!path) return;
+    FILE* f = std::fopen(path, "w");
+    if (!f) return;
+    for (size_t b = 0; b < t.keys.size(); ++b)
+        if (t.choice[b] >= 0) std::fprintf(f, "%s %d\n", t.keys[b].c_str(), t.choice[b]);
+    std::fclose(f);
+}
+int variant_slot(int v) {
+    for (int i = 0; i < 4; ++i)
+        if (GEMM_VARIANTS[i] == v) return i;
+    return -1;
+}
+template <class T> constexpr int type_code() {
+    return Scalar<T>::complex ? (sizeof(T) == 8 ? 2 : 3) : (sizeof(T) == 4 ? 0 : 1);
+}
+bool step_has_variant(const Step& s, int v) {
+    for (int i = 0; i < s.nvar; ++i)
+        if (s.vars[i] == v) return true;
+    return false;
+}
+// the launch takes tile shape v (one of its admissible shapes)
+void set_variant(Step& s, int v) {
+    for (int i = 0; i < s.nvar; ++i)
+        if (s.vars[i] == v) {
+            s.variant = v;
+            s.start = s.vstart[i];
+            s.nwork = s.vtiles[i];
+            return;
+        }
+    throw Error(5, "a GEMM launch was given a tile shape it was not built for");
+}
 
 const char* step_name(int kind) {
     switch (kind) {
@@ -256,43 +382,71 @@ struct StepBuilder {
     // triangular op(A) whose K range the kernel clamps), so only half of the
     // K products count as algorithmic work.
     // a factor update (named and instantiated apart from the TRSM-like products)
-    void update(const std::vector<GemmDesc<T>>& in) {
-        const size_t n0 = steps.size();
-        gemm(in);
-        if (steps.size() > n0) steps.back().update = true;
-    }
+    void update(const std::vector<GemmDesc<T>>& in) { gemm_role(in, false, true); }
 
-    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) {
+    bool solve = false;   // the steps are a solve's (GEMM role 1)
+
+    void gemm(const std::vector<GemmDesc<T>>& in, bool triB = false) { gemm_role(in, triB, false); }
+
+    // A grouped GEMM launch: its tile-start arrays for every admissible tile
+    // shape go to the arena, the shape used is the autotuner's choice for
+    // the launch's bucket or else the cost model's (choose_variant).
+    void gemm_role(const std::vector<GemmDesc<T>>& in, bool triB, bool upd) {
         std::vector<GemmDesc<T>> ds;
-        std::vector<int> start;
-        int total = 0;
-        double flops = 0.0, bytes = 0.0;
-        const int variant = choose_variant(in);
-        for (const auto& d : in)
-            if ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(variant))
-                throw Error(1, "an in-place GEMM needs all of its columns in one tile");
+        double flops = 0.0, bytes = 0.0, ksum = 0.0;
+        bool inplaceWide = false, lower = false;
         for (const auto& d : in) {
-            const int t = gemm_tiles(d.M, d.N, d.flags, variant);
-            if (t == 0) continue;
+            if (d.M <= 0 || d.N <= 0) continue;
             if (d.K == 0 && (d.flags & G_ACCUM)) continue;
             ds.push_back(d);
-            start.push_back(total);
-            total += t;
             const double M = d.M, N = d.N, K = d.K;
             const double entries = (d.flags & G_LOWER) ? M * N - N * (N - 1) / 2 : M * N;
             const double keff = triB ? (K + 1) / 2 : K;
             flops += 2.0 * entries * keff * (Scalar<T>::complex ? 4 : 1);
             bytes += (M * K + K * N + entries * ((d.flags & G_ACCUM) ? 2 : 1)) * sizeof(T);
+            ksum += K;
+            lower = lower || (d.flags & G_LOWER);
         }
         if (ds.empty()) return;
         const bool scaled = ds[0].scaleK != nullptr;
         for (const auto& d : ds)
             if ((d.scaleK != nullptr) != scaled)
                 throw Error(1, "a grouped GEMM mixes scaled and unscaled descriptors");
-        Step s{S_GEMM, arena.add(ds), arena.add(start), (int)ds.size(), total, false, flops, bytes};
-        s.variant = variant;
+        Step s{S_GEMM, arena.add(ds), 0, (int)ds.size(), 0, false, flops, bytes};
         s.scaled = scaled;
+        s.update = upd;
         s.stream = cur;
+        double tiles64 = 0.0;
+        for (const auto& d : ds) tiles64 += gemm_tiles(d.M, d.N, d.flags, TILE_64);
+        for (int v : GEMM_VARIANTS) {
+            if (!gemm_variant_allowed<T>(v)) continue;
+            // an in-place product (C is A or B, the TRSM of the factor) needs
+            // every column of a row block in one tile: a second tile column
+            // would read A after the first one has overwritten it
+            bool ok = true;
+            for (const auto& d : ds)
+                if ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(v)) ok = false;
+            if (!ok) continue;
+            std::vector<int> start;
+            int total = 0;
+            for (const auto& d : ds) {
+                start.push_back(total);
+                total += gemm_tiles(d.M, d.N, d.flags, v);
+            }
+            s.vars[s.nvar] = v;
+            s.vstart[s.nvar] = arena.add(start);
+            s.vtiles[s.nvar] = total;
+            ++s.nvar;
+        }
+        if (s.nvar == 0) throw Error(1, "an in-place GEMM needs all of its columns in one tile");
+        const int role = scaled ? 2 : (upd ? 3 : (solve ? 1 : 0));
+        s.bucket = tune_bucket(type_code<T>(), role, lower, tiles64, ksum / ds.size());
+        const int forced = forced_variant();
+        const int tuned = tuned_variant(s.bucket);
+        const int model = choose_variant(ds);
+        set_variant(s, forced >= 0 && step_has_variant(s, forced)  ? forced
+                       : tuned >= 0 && step_has_variant(s, tuned) ? tuned
+                                                                   : model);
         steps.push_back(s);
     }
 
@@ -1297,8 +1451,102 @@ struct Numeric : NumericBase {
         }
     }
 
+    // one scheduled launch of step s on stream st
+    void launch_one(const Step& s, const Arena& a, SolveSched* S, int k, hipStream_t st) {
+        switch (s.kind) {
+            case S_ZERO:
+                launch_zero_segments<T>(a.at<ZeroSeg<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                        s.nwork, st);
+                break;
+            case S_ZLOWER:
+                launch_zero_lower<T>(a.at<ZeroLowerDesc<T>>(s.desc), a.at<int>(s.start),
+                                     s.ndesc, s.nwork, st);
+                break;
+            case S_EXTEND:
+                launch_extend_add<T>(a.at<ExtendDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                     s.nwork, st);
+                break;
+            case S_DIAG:
+                launch_diag_ldl<T>(a.at<DiagDesc<T>>(s.desc), s.ndesc, P.conjugate,
+                                   P.dStatus.as<int>(), st);
+                break;
+            case S_GEMM:
+                launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
+                               s.variant, s.scaled ? 2 : (s.update ? 3 : (S ? 1 : 0)), st);
+                break;
+            case S_GATHER:
+                launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
+                                 S->wp, S->x1, P.N, k, P.dLow.as<int>(), st);
+                break;
+            case S_REDUCE:
+                launch_reduce_slices<T>(a.at<ReduceDesc<T>>(s.desc), a.at<int>(s.start),
+                                        s.ndesc, s.nwork, st);
+                break;
+            case S_COPY:
+                launch_copy2d<T>(a.at<CopyDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                 s.nwork, st);
+                break;
+            case S_SEXTEND:
+                launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
+                                       s.nwork, S->wp, S->x1, P.N, P.dRel.as<int>(), k, st);
+                break;
+            case S_BK:
+                launch_bk_panel<T>(a.at<BkDesc<T>>(s.desc), s.ndesc, s.variant, P.conjugate,
+                                   P.dStatus.as<int>(), st);
+                break;
+            case S_TRINV:
+                launch_unit_lower_inverse<T>(a.at<TriInvDesc<T>>(s.desc), s.ndesc, st);
+                break;
+            case S_GCOLS:
+                launch_gather_cols<T>(a.at<GatherColsDesc<T>>(s.desc), a.at<int>(s.start),
+                                      s.ndesc, s.nwork, st);
+                break;
+            case S_QSCALE:
+                launch_quasi_scale<T>(a.at<QuasiDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                      s.nwork, P.conjugate, st);
+                break;
+            case S_PERM:
+                // gather (variant 0) or scatter (1) of the fronts' rows from
+                // the first nodal buffer into the second
+                launch_permute_front_rows<T>(a.at<PermSeg>(s.desc), a.at<int>(s.start), s.ndesc,
+                                             s.nwork, S->x1, S->x2, P.N, k, pivpool.as<int>(),
+                                             s.variant != 0, st);
+                break;
+        }
+    }
+
+    // Autotuning pass: every launch on the plan's stream, each grouped GEMM
+    // once per admissible tile shape between events (its results are
+    // meaningless: the products are applied several times).
+    struct TuneSample {
+        int bucket, variant;
+        hipEvent_t e0, e1;
+        int step;   // running index of the launch in the pass
+    };
+    std::vector<TuneSample>* tuning = nullptr;
+    int tuneStep = 0;
+
     void run(const std::vector<Step>& steps, const Arena& a, SolveSched* S, int k) {
         for (const Step& s : steps) {
+            if (tuning) {
+                if (s.kind == S_RECORD || s.kind == S_WAIT) continue;
+                if (s.kind != S_GEMM) {
+                    launch_one(s, a, S, k, P.stream);
+                    continue;
+                }
+                const int id = tuneStep++;
+                for (int i = 0; i < s.nvar; ++i) {
+                    Step t = s;
+                    set_variant(t, s.vars[i]);
+                    TuneSample x{s.bucket, s.vars[i], P.next_event(), nullptr, id};
+                    CQ_HIP(hipEventRecord(x.e0, P.stream));
+                    launch_one(t, a, S, k, P.stream);
+                    x.e1 = P.next_event();
+                    CQ_HIP(hipEventRecord(x.e1, P.stream));
+                    tuning->push_back(x);
+                }
+                continue;
+            }
             // a launch's profiling events go on its own stream: the interval is
             // the kernel's duration as it runs beside the other stream's work
             hipStream_t st = s.stream ? P.stream2 : P.stream;
@@ -1315,66 +1563,7 @@ struct Numeric : NumericBase {
                 e0 = P.next_event();
                 CQ_HIP(hipEventRecord(e0, st));
             }
-            switch (s.kind) {
-                case S_ZERO:
-                    launch_zero_segments<T>(a.at<ZeroSeg<T>>(s.desc), a.at<int>(s.start), s.ndesc,
-                                            s.nwork, st);
-                    break;
-                case S_ZLOWER:
-                    launch_zero_lower<T>(a.at<ZeroLowerDesc<T>>(s.desc), a.at<int>(s.start),
-                                         s.ndesc, s.nwork, st);
-                    break;
-                case S_EXTEND:
-                    launch_extend_add<T>(a.at<ExtendDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
-                                         s.nwork, st);
-                    break;
-                case S_DIAG:
-                    launch_diag_ldl<T>(a.at<DiagDesc<T>>(s.desc), s.ndesc, P.conjugate,
-                                       P.dStatus.as<int>(), st);
-                    break;
-                case S_GEMM:
-                    launch_gemm<T>(a.at<GemmDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                   s.variant, s.scaled ? 2 : (s.update ? 3 : (S ? 1 : 0)), st);
-                    break;
-                case S_GATHER:
-                    launch_gather<T>(a.at<SolveSeg>(s.desc), a.at<int>(s.start), s.ndesc, s.nwork,
-                                     S->wp, S->x1, P.N, k, P.dLow.as<int>(), st);
-                    break;
-                case S_REDUCE:
-                    launch_reduce_slices<T>(a.at<ReduceDesc<T>>(s.desc), a.at<int>(s.start),
-                                            s.ndesc, s.nwork, st);
-                    break;
-                case S_COPY:
-                    launch_copy2d<T>(a.at<CopyDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
-                                     s.nwork, st);
-                    break;
-                case S_SEXTEND:
-                    launch_solve_extend<T>(a.at<SolveExtend>(s.desc), a.at<int>(s.start), s.ndesc,
-                                           s.nwork, S->wp, S->x1, P.N, P.dRel.as<int>(), k, st);
-                    break;
-                case S_BK:
-                    launch_bk_panel<T>(a.at<BkDesc<T>>(s.desc), s.ndesc, s.variant, P.conjugate,
-                                       P.dStatus.as<int>(), st);
-                    break;
-                case S_TRINV:
-                    launch_unit_lower_inverse<T>(a.at<TriInvDesc<T>>(s.desc), s.ndesc, st);
-                    break;
-                case S_GCOLS:
-                    launch_gather_cols<T>(a.at<GatherColsDesc<T>>(s.desc), a.at<int>(s.start),
-                                          s.ndesc, s.nwork, st);
-                    break;
-                case S_QSCALE:
-                    launch_quasi_scale<T>(a.at<QuasiDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
-                                          s.nwork, P.conjugate, st);
-                    break;
-                case S_PERM:
-                    // gather (variant 0) or scatter (1) of the fronts' rows from
-                    // the first nodal buffer into the second
-                    launch_permute_front_rows<T>(a.at<PermSeg>(s.desc), a.at<int>(s.start), s.ndesc,
-                                                 s.nwork, S->x1, S->x2, P.N, k, pivpool.as<int>(),
-                                                 s.variant != 0, st);
-                    break;
-            }
+            launch_one(s, a, S, k, st);
             if (prof) {
                 hipEvent_t e1 = P.next_event();
                 CQ_HIP(hipEventRecord(e1, st));
@@ -1384,9 +1573,10 @@ struct Numeric : NumericBase {
                 // (32x32- and 128x128-tile updates apart: different kernels)
                 const char* name = (s.kind == S_GEMM && !S)
                                        ? ((s.update || s.scaled)
-                                              ? (s.variant == TILE_32    ? "gemm_update_small"
-                                                 : s.variant == TILE_128 ? "gemm_update_large"
-                                                                         : "gemm_update")
+                                              ? (s.variant == TILE_32       ? "gemm_update_small"
+                                                 : s.variant == TILE_128    ? "gemm_update_large"
+                                                 : s.variant == TILE_128_8W ? "gemm_update_w8"
+                                                                            : "gemm_update")
                                               : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
                                                              : step_name(kind);
@@ -1436,8 +1626,7 @@ struct Numeric : NumericBase {
         return true;
     }
 
-    void factor(const V* vals) {
-        hipStream_t st = P.stream;
+    void ensure_factor_schedule() {
         if (block() && !blockBuilt) {
             build_factor_schedule(barena, bsteps, true);
             P.ensure_dep_events(nextEvent);
@@ -1448,6 +1637,11 @@ struct Numeric : NumericBase {
             P.ensure_dep_events(nextEvent);
             pivBuilt = true;
         }
+    }
+
+    void factor(const V* vals) {
+        hipStream_t st = P.stream;
+        ensure_factor_schedule();
         CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
         // host-staged fronts replace their assembled values: converted to
         // the working type on the host, copied in after the assembly
@@ -1535,6 +1729,129 @@ struct Numeric : NumericBase {
             run(S.bwd, S.arena, &S, k);
         };
         if (S.arena.base || !replay(S.graph, body)) body();
+    }
+
+    // The autotuning pass (see TuneTable): the current front type's
+    // factorization of vals and, with rhs > 0, the forward and backward
+    // sweeps of a solve with rhs right-hand sides, every grouped GEMM timed
+    // in each admissible tile shape. Records the fastest shape per bucket,
+    // gives every built schedule of this plan the tuned shapes and drops the
+    // recorded graphs. The fronts hold no factorization afterwards.
+    void autotune(const V* vals, int rhs, int* nBuckets, int* nChanged) {
+        hipStream_t st = P.stream;
+        ensure_factor_schedule();
+        std::vector<TuneSample> samples;
+        tuning = &samples;
+        tuneStep = 0;
+        try {
+            CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
+            launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate,
+                                  st);
+            if (block()) run(bsteps, barena, nullptr, 0);
+            else if (pivoted()) run(psteps, parena, nullptr, 0);
+            else run(fsteps, farena, nullptr, 0);
+            if (rhs > 0) {
+                SolveSched& S = schedule(rhs);
+                run(S.fwd, S.arena, &S, rhs);
+                run(S.bwd, S.arena, &S, rhs);
+            }
+        } catch (...) {
+            tuning = nullptr;
+            throw;
+        }
+        tuning = nullptr;
+        CQ_HIP(hipStreamSynchronize(st));
+        // per launch: milliseconds of each tile shape
+        std::map<int, std::pair<int, std::array<double, 4>>> perStep;   // step -> (bucket, ms)
+        for (const TuneSample& x : samples) {
+            float ms = 0.f;
+            CQ_HIP(hipEventElapsedTime(&ms, x.e0, x.e1));
+            auto it = perStep.find(x.step);
+            if (it == perStep.end())
+                it = perStep.emplace(x.step, std::make_pair(x.bucket, std::array<double, 4>{
+                                                                          -1.0, -1.0, -1.0, -1.0}))
+                         .first;
+            it->second.second[variant_slot(x.variant)] = ms;
+        }
+        P.reset_events();
+        // per bucket: total of each shape over the launches that admit every
+        // shape the bucket's launches have in common
+        std::map<int, std::array<double, 4>> tot;
+        std::map<int, std::array<bool, 4>> common;
+        for (const auto& kv : perStep) {
+            const int b = kv.second.first;
+            const auto& t = kv.second.second;
+            auto ci = common.find(b);
+            if (ci == common.end()) {
+                common[b] = {t[0] >= 0, t[1] >= 0, t[2] >= 0, t[3] >= 0};
+                tot[b] = {0.0, 0.0, 0.0, 0.0};
+            } else {
+                for (int i = 0; i < 4; ++i) ci->second[i] = ci->second[i] && t[i] >= 0;
+            }
+            for (int i = 0; i < 4; ++i)
+                if (t[i] >= 0) tot[b][i] += t[i];
+        }
+        std::map<int, int> before;   // a shape each bucket used before the pass
+        auto note = [&](const std::vector<Step>& steps) {
+            for (const Step& s : steps)
+                if (s.kind == S_GEMM && s.bucket >= 0) before.emplace(s.bucket, s.variant);
+        };
+        note(fsteps);
+        note(bsteps);
+        note(psteps);
+        for (auto& kv : solves) {
+            note(kv.second->fwd);
+            note(kv.second->bwd);
+        }
+        int changed = 0;
+        {
+            TuneTable& tt = tune_table();
+            std::lock_guard<std::mutex> lock(tt.mu);
+            for (const auto& kv : tot) {
+                const int b = kv.first;
+                int best = -1;
+                for (int i = 0; i < 4; ++i)
+                    if (common[b][i] && (best < 0 || kv.second[i] < kv.second[best])) best = i;
+                if (best < 0) continue;
+                tt.choice[b] = GEMM_VARIANTS[best];
+                for (int i = 0; i < 4; ++i) tt.ms[b][i] = common[b][i] ? kv.second[i] : -1.0;
+                auto bi = before.find(b);
+                if (bi != before.end() && bi->second != GEMM_VARIANTS[best]) ++changed;
+            }
+            tune_save_locked(tt);
+        }
+        apply_tuning();
+        *nBuckets = (int)tot.size();
+        *nChanged = changed;
+    }
+
+    // Every built schedule of this plan takes the tuned tile shapes.
+    void apply_tuning() {
+        auto apply = [](std::vector<Step>& steps) {
+            for (Step& s : steps) {
+                if (s.kind != S_GEMM) continue;
+                const int v = tuned_variant(s.bucket);
+                if (v >= 0 && step_has_variant(s, v)) set_variant(s, v);
+            }
+        };
+        apply(fsteps);
+        apply(bsteps);
+        apply(psteps);
+        for (auto& kv : solves) {
+            apply(kv.second->fwd);
+            apply(kv.second->bwd);
+            drop(kv.second->graph);
+        }
+        for (auto& kv : noallocSolves) {
+            apply(kv.second->fwd);
+            apply(kv.second->bwd);
+        }
+        for (Graph& g : fgraph) drop(g);
+    }
+    static void drop(Graph& g) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.tried = false;
     }
 
     V* scratch_cols(int k) {
@@ -1991,6 +2308,34 @@ void Plan::set_front(int s, const void* hL) {
     // is kept zero, as the assembly leaves it
     const int m = fronts[s].m, n = fronts[s].n;
     for (int j = 1; j < n; ++j) std::memset(buf.data() + (size_t)j * m * vb, 0, (size_t)j * vb);
+}
+
+void Plan::autotune(const void* d_vals, int prec, int ftype, int rhs, int* nBuckets,
+                    int* nChanged) {
+    if (rhs < 0 || rhs > 65535) throw Error(1, "bad right-hand-side count for tuning");
+    factor(d_vals, prec, ftype);
+    dispatch(complex, precision, [&](auto* tag) {
+        using T = std::remove_pointer_t<decltype(tag)>;
+        using V = typename Scalar<T>::wide;
+        static_cast<Numeric<T>*>(num.get())->autotune(dVals.as<V>(), rhs, nBuckets, nChanged);
+    });
+    factored = false;
+    factor(d_vals, prec, ftype);
+}
+
+std::string tune_table_text() {
+    TuneTable& t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    tune_load_locked(t);
+    std::string out;
+    char line[256];
+    for (size_t b = 0; b < t.keys.size(); ++b) {
+        if (t.choice[b] < 0) continue;
+        std::snprintf(line, sizeof(line), "%s %d %.4f %.4f %.4f %.4f\n", t.keys[b].c_str(),
+                      t.choice[b], t.ms[b][0], t.ms[b][1], t.ms[b][2], t.ms[b][3]);
+        out += line;
+    }
+    return out;
 }
 
 void Plan::get_pivots(int s, void* hsub, int* hpiv) {

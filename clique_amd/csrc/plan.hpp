@@ -53,6 +53,12 @@ struct NumericBase {
     virtual size_t device_bytes() const = 0;
 };
 
+// The autotuner's table as text, one line per tuned bucket: the bucket
+// (t<type>.r<role>.l<lower>.T<log2 tiles>.K<log2 mean K>), the chosen tile
+// shape and the milliseconds each shape (32x32, 64x64, 128x128 four-wave,
+// 128x128 eight-wave) took over the bucket's launches, -1 where not built.
+std::string tune_table_text();
+
 class Plan {
 public:
     Plan(std::shared_ptr<const Symbolic> sym, const int64_t* rowOff, const int* cols,
@@ -126,7 +132,15 @@ public:
     std::vector<Pending> pending;
     std::map<std::string, ProfileEntry> profile;
     hipEvent_t next_event();
+    void reset_events() { evNext_ = 0; }
     void resolve_profile();
+    // Times every admissible GEMM tile shape of the factorization (front
+    // type ftype, precision prec) of d_vals and, with rhs > 0, of a solve
+    // with rhs right-hand sides, records the fastest per launch bucket in the
+    // process-wide tuning table and factors d_vals again with the tuned
+    // shapes. nBuckets: buckets measured; nChanged: buckets whose shape
+    // changed.
+    void autotune(const void* d_vals, int prec, int ftype, int rhs, int* nBuckets, int* nChanged);
 
 private:
     double lastFactorMs_ = 0.0;

@@ -234,6 +234,30 @@ cliqg_status cliqg_plan_get_front_pivots(cliqg_plan plan, int s, void* h_subdiag
  * frontTree.localFronts[s].frontL before LDL (tests/Refactor.cpp:150-160). */
 cliqg_status cliqg_plan_set_front(cliqg_plan plan, int s, const void* h_L);
 
+/* GEMM tile autotuner (no reference counterpart; Elemental's dense calls have
+ * fixed blockings). Factors d_vals with the given precision and front type
+ * while timing every grouped GEMM launch in each tile shape it admits
+ * (32x32, 64x64, 128x128 of four waves for real types, 128x128 of eight
+ * waves for complex FP32) and, with solve_rhs > 0, does the same for a solve
+ * with solve_rhs right-hand sides; the fastest shape per launch bucket
+ * (working type, role, lower masking, log2 of the launch's 64x64 tile count
+ * and of its mean K) goes into a table shared by every plan of the process,
+ * and every schedule built afterwards uses it. The plan is then factored
+ * again with the tuned shapes (as cliqg_plan_factor). h_buckets: buckets
+ * measured, h_changed: buckets whose shape changed (either may be NULL).
+ * With the environment variable CLIQG_TUNE_CACHE=<file> the table is read
+ * from that file at first use and written back after each tuning. */
+cliqg_status cliqg_plan_autotune(cliqg_plan plan, const void* d_vals, cliqg_precision precision,
+                                 cliqg_front_type front_type, int solve_rhs, int* h_buckets,
+                                 int* h_changed);
+/* The tuning table as text, one line per tuned bucket: bucket, chosen shape
+ * (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave) and the
+ * milliseconds of the 32x32, 64x64, 128x128 and eight-wave 128x128 shapes
+ * over the bucket's launches (-1 where not built). h_length gets the full
+ * length; at most capacity - 1 characters and a terminating zero are
+ * copied to h_buf (which may be NULL to query the length). */
+cliqg_status cliqg_autotune_table(char* h_buf, size_t capacity, size_t* h_length);
+
 /* Timing of the last factorization on the plan's stream (HIP events). */
 cliqg_status cliqg_plan_last_factor_ms(cliqg_plan plan, double* ms);
 
