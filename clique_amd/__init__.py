@@ -29,8 +29,9 @@ from ._lib import CliqgError, build, lib  # noqa: F401
 LDL_1D = LDL_2D = "ldl"
 BLOCK_LDL_1D = BLOCK_LDL_2D = "block_ldl"
 LDL_INTRAPIV_1D = LDL_INTRAPIV_2D = "ldl_intrapiv"
+BLOCK_LDL_INTRAPIV_1D = BLOCK_LDL_INTRAPIV_2D = "block_ldl_intrapiv"
 
-_FRONT_TYPES = {"ldl": 0, "block_ldl": 1, "ldl_intrapiv": 2}
+_FRONT_TYPES = {"ldl": 0, "block_ldl": 1, "ldl_intrapiv": 2, "block_ldl_intrapiv": 3}
 _PRECISIONS = {"f64": 0, "fp64": 0, "double": 0, "f32": 1, "fp32": 1, "single": 1}
 
 
@@ -147,6 +148,22 @@ def NaturalNestedDissection(nx, ny, nz, indptr, indices, cutoff=128):
     h = ctypes.c_void_p()
     _lib.check(lib().cliqg_symbolic_natural(
         int(nx), int(ny), int(nz), N, ip.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        ind.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(cutoff), ctypes.byref(h)))
+    return Symbolic(h, N)
+
+
+def NestedDissection(indptr, indices, cutoff=128):
+    """Nested dissection of a general graph given as CSR (the reference's
+    NestedDissection, symbolic/nested_dissection.hpp:491-533, with this
+    library's own bisection in place of METIS)."""
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ind = np.ascontiguousarray(indices, dtype=np.int32)
+    N = ip.size - 1
+    if N <= 0 or ind.size < ip[-1]:
+        raise ValueError("malformed CSR arrays")
+    h = ctypes.c_void_p()
+    _lib.check(lib().cliqg_symbolic_graph(
+        N, ip.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
         ind.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), int(cutoff), ctypes.byref(h)))
     return Symbolic(h, N)
 
@@ -273,7 +290,7 @@ class DistSymmFrontTree:
         d = np.zeros(max(n, 1), dtype=dt)
         _lib.check(lib().cliqg_plan_get_front(self._h, s, L.ctypes.data_as(ctypes.c_void_p),
                                               d.ctypes.data_as(ctypes.c_void_p)))
-        if self.front_type == "block_ldl":
+        if self.front_type in ("block_ldl", "block_ldl_intrapiv"):
             # block fronts: ATL holds inv(L11 D L11^T), ABL the original A21, no diagonal
             return L, d[:0]
         return L, d[:n]

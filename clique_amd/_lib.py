@@ -51,7 +51,7 @@ EXPORTS = [
     "cliqg_plan_last_factor_ms", "cliqg_plan_set_profiling", "cliqg_plan_get_profile",
     "cliqg_plan_set_graphs", "cliqg_plan_solve_noalloc", "cliqg_plan_set_front",
     "cliqg_permute_rows", "cliqg_col_norms", "cliqg_plan_get_front_pivots",
-    "cliqg_plan_autotune", "cliqg_autotune_table",
+    "cliqg_plan_autotune", "cliqg_autotune_table", "cliqg_symbolic_graph",
 ]
 
 
@@ -78,6 +78,7 @@ def lib():
         "cliqg_symbolic_natural": (c_int, [c_int, c_int, c_int, c_int, i64p, ip, c_int,
                                            ctypes.POINTER(vp)]),
         "cliqg_symbolic_concat": (c_int, [ctypes.POINTER(vp), c_int, ctypes.POINTER(vp)]),
+        "cliqg_symbolic_graph": (c_int, [c_int, i64p, ip, c_int, ctypes.POINTER(vp)]),
         "cliqg_symbolic_destroy": (c_int, [vp]),
         "cliqg_symbolic_num_nodes": (c_int, [vp, ip, ip]),
         "cliqg_symbolic_node": (c_int, [vp, c_int, ip]),

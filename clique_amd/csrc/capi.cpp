@@ -102,6 +102,17 @@ cliqg_status cliqg_symbolic_natural(int nx, int ny, int nz, int N, const int64_t
     });
 }
 
+cliqg_status cliqg_symbolic_graph(int N, const int64_t* h_row_off, const int* h_cols, int cutoff,
+                                  cliqg_symbolic* out) {
+    return guarded([&] {
+        require(out && h_row_off && h_cols && N > 0, "null argument");
+        auto sym = std::make_shared<cliqg::Symbolic>();
+        const cliqg::Graph g = cliqg::graph_from_csr(N, h_row_off, h_cols);
+        cliqg::graph_nested_dissection(g, cutoff, *sym);
+        *out = new cliqg_symbolic_s{std::move(sym)};
+    });
+}
+
 cliqg_status cliqg_symbolic_concat(const cliqg_symbolic* syms, int count, cliqg_symbolic* out) {
     return guarded([&] {
         require(out && syms && count > 0, "null argument");

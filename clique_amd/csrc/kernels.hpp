@@ -192,6 +192,26 @@ struct QuasiDesc {
     int64_t ld;
     int rows, cols;
 };
+// Z = D^{-1} X for an n x n block X (rows of one or two, D block diagonal)
+template <class T>
+struct QuasiRowsDesc {
+    const T* X;
+    T* Z;
+    const void* D;
+    const void* sub;
+    int64_t ldx, ldz;
+    int n;
+};
+// A(piv[i], piv[j]) = Y(i, j) for i, j < n (PermuteRows and PermuteCols of
+// a symmetric block)
+template <class T>
+struct PermSymDesc {
+    const T* Y;
+    T* A;
+    const int* piv;
+    int64_t ldy, lda;
+    int n;
+};
 // A front's rows [off, off + n) of the nodal right-hand sides
 struct PermSeg {
     int off, n;
@@ -218,6 +238,13 @@ void launch_permute_front_rows(const PermSeg* d_segs, const int* d_row_start, in
 template <class T, class V>
 void launch_quasi_diag_solve(T* xn, const V* D, const V* sub, int N, int k, bool herm,
                              int* d_status, hipStream_t s);
+// start[i] = workgroups before descriptor i (one per column and 256-row chunk)
+template <class T>
+void launch_quasi_rows(const QuasiRowsDesc<T>* d_descs, const int* d_start, int ndesc,
+                       int nblocks, bool herm, hipStream_t s);
+template <class T>
+void launch_permute_sym(const PermSymDesc<T>* d_descs, const int* d_start, int ndesc, int nblocks,
+                        hipStream_t s);
 
 // ---- launchers (stream-ordered, asynchronous) ----------------------------------
 // Clears every segment; chunk_start[i] = number of 1024-element chunks before

@@ -400,6 +400,50 @@ __global__ void k_quasi_diag_solve(T* __restrict__ xn, const V* __restrict__ D,
     }
 }
 
+// Z(i, c) = (D^{-1} X)(i, c): the block-LDL inverse's middle factor with
+// 2x2 pivots (Trdtrmm with the quasi-diagonal D, ldl/local_front_block.hpp:53)
+template <class T, bool HERM>
+__global__ __launch_bounds__(256) void k_quasi_rows(const QuasiRowsDesc<T>* __restrict__ descs,
+                                                    const int* __restrict__ start, int ndesc) {
+    using V = typename Scalar<T>::wide;
+    const int g = blockIdx.x;
+    const int di = segment_of(start, ndesc, g);
+    const QuasiRowsDesc<T> d = descs[di];
+    const int chunks = (d.n + 255) / 256;
+    const int u = g - start[di];
+    const int c = u / chunks, i = (u % chunks) * 256 + threadIdx.x;
+    if (i >= d.n) return;
+    const V* D = reinterpret_cast<const V*>(d.D);
+    const V* S = reinterpret_cast<const V*>(d.sub);
+    const bool first = nonzero_v(S[i]);
+    const bool second = i > 0 && nonzero_v(S[i - 1]);
+    V out;
+    if (first || second) {
+        const int r0 = first ? i : i - 1;
+        const V d11 = D[r0], d22 = D[r0 + 1], d21 = S[r0], d12 = cj(d21, HERM);
+        const V idet = recip(d11 * d22 - d12 * d21);
+        const V a = cvt<V>(d.X[r0 + (int64_t)c * d.ldx]), b = cvt<V>(d.X[r0 + 1 + (int64_t)c * d.ldx]);
+        out = first ? (d22 * a - d12 * b) * idet : (d11 * b - d21 * a) * idet;
+    } else {
+        out = cvt<V>(d.X[i + (int64_t)c * d.ldx]) * recip(D[i]);
+    }
+    d.Z[i + (int64_t)c * d.ldz] = cvt<T>(out);
+}
+
+// A(piv[i], piv[j]) = Y(i, j)
+template <class T>
+__global__ __launch_bounds__(256) void k_permute_sym(const PermSymDesc<T>* __restrict__ descs,
+                                                     const int* __restrict__ start, int ndesc) {
+    const int g = blockIdx.x;
+    const int di = segment_of(start, ndesc, g);
+    const PermSymDesc<T> d = descs[di];
+    const int chunks = (d.n + 255) / 256;
+    const int u = g - start[di];
+    const int j = u / chunks, i = (u % chunks) * 256 + threadIdx.x;
+    if (i >= d.n) return;
+    d.A[d.piv[i] + (int64_t)d.piv[j] * d.lda] = d.Y[i + (int64_t)j * d.ldy];
+}
+
 inline unsigned blocks_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -467,6 +511,25 @@ void launch_quasi_diag_solve(T* xn, const V* D, const V* sub, int N, int k, bool
                            xn, D, sub, N, k, d_status);
 }
 
+template <class T>
+void launch_quasi_rows(const QuasiRowsDesc<T>* d_descs, const int* d_start, int ndesc,
+                       int nblocks, bool herm, hipStream_t s) {
+    if (ndesc <= 0 || nblocks <= 0) return;
+    if (herm)
+        hipLaunchKernelGGL((k_quasi_rows<T, true>), dim3(nblocks), dim3(256), 0, s, d_descs, d_start,
+                           ndesc);
+    else
+        hipLaunchKernelGGL((k_quasi_rows<T, false>), dim3(nblocks), dim3(256), 0, s, d_descs,
+                           d_start, ndesc);
+}
+
+template <class T>
+void launch_permute_sym(const PermSymDesc<T>* d_descs, const int* d_start, int ndesc, int nblocks,
+                        hipStream_t s) {
+    if (ndesc <= 0 || nblocks <= 0) return;
+    hipLaunchKernelGGL(k_permute_sym<T>, dim3(nblocks), dim3(256), 0, s, d_descs, d_start, ndesc);
+}
+
 #define CQ_INST_PIV(T, V)                                                                        \
     template void launch_bk_panel<T>(const BkDesc<T>*, int, int, bool, int*, hipStream_t);       \
     template void launch_unit_lower_inverse<T>(const TriInvDesc<T>*, int, hipStream_t);          \
@@ -477,7 +540,10 @@ void launch_quasi_diag_solve(T* xn, const V* D, const V* sub, int N, int k, bool
     template void launch_permute_front_rows<T>(const PermSeg*, const int*, int, int, const T*,   \
                                                T*, int, int, const int*, bool, hipStream_t);     \
     template void launch_quasi_diag_solve<T, V>(T*, const V*, const V*, int, int, bool, int*,    \
-                                                hipStream_t);
+                                                hipStream_t);                                    \
+    template void launch_quasi_rows<T>(const QuasiRowsDesc<T>*, const int*, int, int, bool,     \
+                                       hipStream_t);                                             \
+    template void launch_permute_sym<T>(const PermSymDesc<T>*, const int*, int, int, hipStream_t);
 
 CQ_INST_PIV(float, double)
 CQ_INST_PIV(double, double)

@@ -145,7 +145,8 @@ struct IntervalAlloc {
 };
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
-                S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER, S_BK, S_TRINV, S_GCOLS, S_QSCALE, S_PERM };
+                S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER, S_BK, S_TRINV, S_GCOLS, S_QSCALE, S_PERM,
+                S_QROWS, S_PSYM };
 
 struct Step {
     int kind;
@@ -303,6 +304,8 @@ const char* step_name(int kind) {
         case 15: return "permute_cols";
         case 16: return "quasi_scale";
         case 17: return "solve_permute";
+        case 18: return "quasi_rows";
+        case 19: return "permute_sym";
         default: return "other";
     }
 }
@@ -528,19 +531,18 @@ struct Numeric : NumericBase {
     // W = L*D below the diagonal blocks of the current level's fronts (the
     // TRSM writes it next to L), the left operand of every update
     DevBuf wbuf;
-    Arena farena, barena;
-    std::vector<Step> fsteps, bsteps;   // LDL and block-LDL factor schedules
+    // factor schedules by front type (cliqg_front_type: LDL, block LDL,
+    // pivoted LDL, pivoted block LDL), built at first use
+    Arena farenas[4];
+    std::vector<Step> fsched[4];
+    bool fbuilt[4] = {false, false, false, false};
     int nextEvent = 0;                  // dependency events used by the schedules
-    bool blockBuilt = false;
     // block LDL work buffers, reused level by level: the fronts' original
     // A21, the unit-lower inverses of L11 and the n x 64 product panels
     DevBuf bsave, bsinv, btmp;
     // pivoted factorization (LDL_INTRAPIV): D's subdiagonal and each front's
     // pivot permutation, in nodal order, and its schedule
     DevBuf subpool, pivpool;
-    Arena parena;
-    std::vector<Step> psteps;
-    bool pivBuilt = false;
 
     // A captured launch sequence (HIP graph), replayed instead of the
     // individual launches.
@@ -567,7 +569,7 @@ struct Numeric : NumericBase {
         int64_t wsUsed = 0;      // split-K workspace entries the schedule uses
         Graph graph;   // forward, diagonal and backward sweeps
     };
-    Graph fgraph[3];   // LDL, block-LDL and pivoted LDL factorizations
+    Graph fgraph[4];   // the factorizations, by front type
     std::map<int, std::unique_ptr<SolveSched>> solves;
     // schedules of the allocation-free solve (descriptors into the scratch array)
     std::map<int, std::unique_ptr<SolveSched>> noallocSolves;
@@ -599,18 +601,19 @@ struct Numeric : NumericBase {
             }
             wbuf.alloc(std::max<int64_t>(mw, 1) * sizeof(T));
         }
-        build_factor_schedule(farena, fsteps, false);
+        build_factor_schedule(farenas[0], fsched[0], false);
+        fbuilt[0] = true;
         P.ensure_dep_events(nextEvent);
     }
 
-    bool block() const { return P.frontType == 1; }
-    bool pivoted() const { return P.frontType == 2; }
+    bool block() const { return P.frontType == 1 || P.frontType == 3; }
+    bool pivoted() const { return P.frontType == 2 || P.frontType == 3; }
 
     size_t device_bytes() const override {
-        size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + farena.dev.bytes +
-                   barena.dev.bytes + bsave.bytes + bsinv.bytes + btmp.bytes + rbuf.bytes +
-                   identity.bytes + linvw.bytes + wtmp.bytes + wbuf.bytes + subpool.bytes +
-                   pivpool.bytes + parena.dev.bytes;
+        size_t b = fpool.bytes + upool.bytes + dpool.bytes + linv.bytes + bsave.bytes +
+                   bsinv.bytes + btmp.bytes + rbuf.bytes + identity.bytes + linvw.bytes +
+                   wtmp.bytes + wbuf.bytes + subpool.bytes + pivpool.bytes;
+        for (const Arena& a : farenas) b += a.dev.bytes;
         for (auto& kv : solves)
             b += kv.second->arena.dev.bytes + kv.second->wpool.bytes + kv.second->xn.bytes +
                  kv.second->xn2.bytes + kv.second->tmp.bytes + kv.second->splitWs.bytes;
@@ -839,7 +842,7 @@ struct Numeric : NumericBase {
             }
             sb.update(syrk);
             extend(true);
-            if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp);
+            if (blockLdl) block_inverse(sb, lv, Sv, Xi, Tp, pivot, Wf);
             else wide_inverse(sb, lv);
         }
         arena.finish();
@@ -1014,8 +1017,13 @@ struct Numeric : NumericBase {
         }
     }
 
-    template <class FS, class FX, class FT>
-    void block_inverse(StepBuilder<T>& sb, const std::vector<int>& lv, FS Sv, FX Xi, FT Tp) {
+    // With pivoting (FrontBlockLDL's intraPiv branch, ldl/local_front_block.hpp:
+    // 35-57) the last product is ATL := P^T (X^T D^{-1} X) P with D's 2x2
+    // blocks: Z = D^{-1} X into the dead L11, Y = op(X)^T Z into the W
+    // buffer, then ATL(piv[i], piv[j]) = Y(i, j).
+    template <class FS, class FX, class FT, class FW>
+    void block_inverse(StepBuilder<T>& sb, const std::vector<int>& lv, FS Sv, FX Xi, FT Tp,
+                       bool pivot, FW Wf) {
         const int cA = P.conjugate ? G_CONJA : 0;
         std::vector<ZeroSeg<T>> zs;
         int maxq = 0;
@@ -1051,14 +1059,35 @@ struct Numeric : NumericBase {
         }
         std::vector<GemmDesc<T>> inv;
         std::vector<CopyDesc<T>> restore;
+        std::vector<QuasiRowsDesc<T>> qr;
+        std::vector<PermSymDesc<T>> ps;
+        std::vector<int> qstart;
+        int qblocks = 0;
         for (int s : lv) {
             const FrontLayout& f = P.fronts[s];
             if (f.n == 0) continue;
-            inv.push_back(GemmDesc<T>{Xi(s), Xi(s), L(s), D(s), nullptr, f.n, f.n, f.m, f.n, f.n,
-                                      f.n, G_TRANSA | G_LOWER | G_SYMM | G_INVK | G_KLO | cA});
+            if (pivot) {
+                qr.push_back(QuasiRowsDesc<T>{Xi(s), L(s), D(s), Sub(s), f.n, f.m, f.n});
+                inv.push_back(GemmDesc<T>{Xi(s), L(s), Wf(s), nullptr, nullptr, f.n, f.m, f.m, f.n,
+                                          f.n, f.n, G_TRANSA | G_KLO | cA});
+                ps.push_back(PermSymDesc<T>{Wf(s), L(s), Piv(s), f.m, f.m, f.n});
+                qstart.push_back(qblocks);
+                qblocks += f.n * ((f.n + 255) / 256);
+            } else {
+                inv.push_back(GemmDesc<T>{Xi(s), Xi(s), L(s), D(s), nullptr, f.n, f.n, f.m, f.n,
+                                          f.n, f.n,
+                                          G_TRANSA | G_LOWER | G_SYMM | G_INVK | G_KLO | cA});
+            }
             restore.push_back(CopyDesc<T>{Sv(s), L(s) + f.n, f.u, f.m, f.u, f.n});
         }
-        sb.gemm(inv, true);
+        if (pivot && !qr.empty()) {
+            const size_t st = sb.arena.add(qstart);
+            sb.steps.push_back(Step{S_QROWS, sb.arena.add(qr), st, (int)qr.size(), qblocks, false});
+            sb.gemm(inv, true);
+            sb.steps.push_back(Step{S_PSYM, sb.arena.add(ps), st, (int)ps.size(), qblocks, false});
+        } else {
+            sb.gemm(inv, true);
+        }
         sb.copy(restore);
     }
 
@@ -1341,7 +1370,7 @@ struct Numeric : NumericBase {
                         steps.push_back(Step{S_ZERO, S.arena.add(zs), S.arena.add(zst),
                                              (int)zs.size(), chunks, false});
                     sextend(false);
-                    if (pivoted()) permute(false);
+                    if (pivoted() && !blk) permute(false);
                     if (blk) {
                         // FrontBlockLowerForwardSolve (lower_solve/local_front_block.hpp:28-48):
                         // X_s := inv(ATL) X_s through a copy of X_s
@@ -1445,7 +1474,7 @@ struct Numeric : NumericBase {
                         sb.gemm(tr, true);
                         sb.gemm(up);
                     }
-                    if (pivoted()) permute(true);
+                    if (pivoted() && !blk) permute(true);
                 }
             }
         }
@@ -1504,6 +1533,14 @@ struct Numeric : NumericBase {
             case S_QSCALE:
                 launch_quasi_scale<T>(a.at<QuasiDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
                                       s.nwork, P.conjugate, st);
+                break;
+            case S_QROWS:
+                launch_quasi_rows<T>(a.at<QuasiRowsDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                     s.nwork, P.conjugate, st);
+                break;
+            case S_PSYM:
+                launch_permute_sym<T>(a.at<PermSymDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                      s.nwork, st);
                 break;
             case S_PERM:
                 // gather (variant 0) or scatter (1) of the fronts' rows from
@@ -1627,15 +1664,11 @@ struct Numeric : NumericBase {
     }
 
     void ensure_factor_schedule() {
-        if (block() && !blockBuilt) {
-            build_factor_schedule(barena, bsteps, true);
+        const int ft = P.frontType;
+        if (!fbuilt[ft]) {
+            build_factor_schedule(farenas[ft], fsched[ft], block(), pivoted());
             P.ensure_dep_events(nextEvent);
-            blockBuilt = true;
-        }
-        if (pivoted() && !pivBuilt) {
-            build_factor_schedule(parena, psteps, false, true);
-            P.ensure_dep_events(nextEvent);
-            pivBuilt = true;
+            fbuilt[ft] = true;
         }
     }
 
@@ -1658,12 +1691,7 @@ struct Numeric : NumericBase {
                 CQ_HIP(hipMemcpyAsync(L(kv.first), staged.back().data(), mn * sizeof(T),
                                       hipMemcpyHostToDevice, st));
             }
-            if (block())
-                run(bsteps, barena, nullptr, 0);
-            else if (pivoted())
-                run(psteps, parena, nullptr, 0);
-            else
-                run(fsteps, farena, nullptr, 0);
+            run(fsched[P.frontType], farenas[P.frontType], nullptr, 0);
         };
         if (!P.stagedFronts.empty() || !replay(fgraph[P.frontType], body)) body();
         // the copies read the host buffers until they complete
@@ -1721,7 +1749,7 @@ struct Numeric : NumericBase {
             run(S.fwd, S.arena, &S, k);
             // block fronts hold inv(ATL), so there is no separate diagonal
             // solve (solve.hpp:48-53)
-            if (pivoted())
+            if (pivoted() && !block())
                 launch_quasi_diag_solve<T, V>(S.x2, dpool.as<V>(), subpool.as<V>(), P.N, k,
                                               P.conjugate, S.status, P.stream);
             else if (!block())
@@ -1747,9 +1775,7 @@ struct Numeric : NumericBase {
             CQ_HIP(hipMemsetAsync(fpool.p, 0, fpool.bytes, st));
             launch_assemble<T, V>(vals, P.dAsmDst.as<int64_t>(), P.nnz, fpool.as<T>(), P.conjugate,
                                   st);
-            if (block()) run(bsteps, barena, nullptr, 0);
-            else if (pivoted()) run(psteps, parena, nullptr, 0);
-            else run(fsteps, farena, nullptr, 0);
+            run(fsched[P.frontType], farenas[P.frontType], nullptr, 0);
             if (rhs > 0) {
                 SolveSched& S = schedule(rhs);
                 run(S.fwd, S.arena, &S, rhs);
@@ -1796,9 +1822,7 @@ struct Numeric : NumericBase {
             for (const Step& s : steps)
                 if (s.kind == S_GEMM && s.bucket >= 0) before.emplace(s.bucket, s.variant);
         };
-        note(fsteps);
-        note(bsteps);
-        note(psteps);
+        for (const auto& fs : fsched) note(fs);
         for (auto& kv : solves) {
             note(kv.second->fwd);
             note(kv.second->bwd);
@@ -1834,9 +1858,7 @@ struct Numeric : NumericBase {
                 if (v >= 0 && step_has_variant(s, v)) set_variant(s, v);
             }
         };
-        apply(fsteps);
-        apply(bsteps);
-        apply(psteps);
+        for (auto& fs : fsched) apply(fs);
         for (auto& kv : solves) {
             apply(kv.second->fwd);
             apply(kv.second->bwd);
@@ -2114,7 +2136,7 @@ static void dispatch(bool complex, int precision, F&& f) {
 void Plan::factor(const void* d_vals, int prec, int ftype) {
     after_caller();
     if (prec != 0 && prec != 1) throw Error(1, "unknown precision");
-    if (ftype != 0 && ftype != 1 && ftype != 2) throw Error(1, "unknown front type");
+    if (ftype < 0 || ftype > 3) throw Error(1, "unknown front type");
     if (!num || prec != precision) {
         num.reset();
         precision = prec;

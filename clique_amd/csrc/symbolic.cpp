@@ -155,6 +155,199 @@ void build_children(const Graph& g, const std::vector<int>& perm,
     build(right, R, L);
 }
 
+// Vertex bisection of a general graph (the role METIS plays in the
+// reference's Bisect, nested_dissection.hpp:535-596; METIS itself is not
+// used): a level structure rooted at a pseudo-peripheral vertex (George and
+// Liu: repeated breadth-first searches from a vertex of least degree in the
+// deepest level until the depth stops growing); the level that best balances
+// the vertices before and after it is the separator; separator vertices
+// without neighbours on one side move to the other side. Vertices the search
+// does not reach (other components) join the smaller side. Only edges inside
+// the subgraph (targets < n) count. When no component holds half of the
+// vertices, whole components are split between the sides. Returns the
+// separator size; perm puts
+// the left part first, then the right part, then the separator, each in
+// increasing vertex order, as the natural bisection does.
+int graph_bisect(const Graph& g, int& L, int& R, std::vector<int>& perm) {
+    const int n = g.n;
+    std::vector<int> level(n, -1), order, comp(n, -1);
+    order.reserve(n);
+    // breadth-first search from root inside its component; returns the depth
+    auto bfs = [&](int root) {
+        for (int s : order) level[s] = -1;
+        order.clear();
+        level[root] = 0;
+        order.push_back(root);
+        for (size_t h = 0; h < order.size(); ++h) {
+            const int s = order[h];
+            for (int64_t k = g.off[s]; k < g.off[s + 1]; ++k) {
+                const int t = g.tgt[k];
+                if (t < n && t != s && level[t] < 0) {
+                    level[t] = level[s] + 1;
+                    order.push_back(t);
+                }
+            }
+        }
+        return level[order.back()];
+    };
+    auto degree = [&](int s) {
+        int d = 0;
+        for (int64_t k = g.off[s]; k < g.off[s + 1]; ++k) d += (g.tgt[k] < n && g.tgt[k] != s);
+        return d;
+    };
+    // connected components; the largest one is bisected, the others join a side
+    std::vector<int> compSize, compRoot;
+    for (int s = 0; s < n; ++s) {
+        if (comp[s] >= 0) continue;
+        bfs(s);
+        int best = s;
+        for (int t : order) {
+            comp[t] = (int)compSize.size();
+            if (degree(t) < degree(best)) best = t;
+        }
+        compSize.push_back((int)order.size());
+        compRoot.push_back(best);
+    }
+    int big = 0;
+    for (int c = 1; c < (int)compSize.size(); ++c)
+        if (compSize[c] > compSize[big]) big = c;
+    // part: 0 left, 1 right, 2 separator, 3 undecided (other components)
+    std::vector<char> part(n, 3);
+    if (2 * compSize[big] >= n) {
+        // pseudo-peripheral root: repeated searches from a vertex of least
+        // degree in the deepest level while the depth grows
+        int root = compRoot[big];
+        int depth = bfs(root);
+        for (int it = 0; it < 8; ++it) {
+            int cand = -1;
+            for (int s : order)
+                if (level[s] == depth && (cand < 0 || degree(s) < degree(cand))) cand = s;
+            const int d = bfs(cand);
+            if (d <= depth) {
+                depth = bfs(root);
+                break;
+            }
+            root = cand;
+            depth = d;
+        }
+        const int reached = (int)order.size();
+        std::vector<int> count(depth + 1, 0);
+        for (int s : order) ++count[level[s]];
+        // separator level: the one minimising |before - after| (ties to the
+        // smaller level), strictly inside the structure when it has one
+        const int lo = depth >= 2 ? 1 : 0, hi = depth >= 2 ? depth - 1 : depth;
+        int sepLevel = lo, before = 0, bestGap = -1;
+        for (int l = 0; l <= depth; ++l) {
+            if (l >= lo && l <= hi) {
+                const int after = reached - before - count[l];
+                const int gap = before > after ? before - after : after - before;
+                if (bestGap < 0 || gap < bestGap) {
+                    bestGap = gap;
+                    sepLevel = l;
+                }
+            }
+            before += count[l];
+        }
+        for (int s : order) part[s] = level[s] < sepLevel ? 0 : (level[s] > sepLevel ? 1 : 2);
+        for (int s : order) {
+            if (part[s] != 2) continue;
+            bool toL = false, toR = false;
+            for (int64_t k = g.off[s]; k < g.off[s + 1]; ++k) {
+                const int t = g.tgt[k];
+                if (t >= n || t == s) continue;
+                toL = toL || part[t] == 0;
+                toR = toR || part[t] == 1;
+            }
+            if (!toR && toL) part[s] = 0;
+            else if (!toL && toR) part[s] = 1;
+        }
+    }
+    // the other components, largest first, each to the smaller side
+    int cnt[3] = {0, 0, 0};
+    for (int s = 0; s < n; ++s)
+        if (part[s] < 3) ++cnt[(int)part[s]];
+    std::vector<int> rest;
+    for (int c = 0; c < (int)compSize.size(); ++c)
+        if (!(2 * compSize[big] >= n && c == big)) rest.push_back(c);
+    std::stable_sort(rest.begin(), rest.end(),
+                     [&](int a, int b) { return compSize[a] > compSize[b]; });
+    std::vector<char> side(compSize.size(), 0);
+    for (int c : rest) {
+        side[c] = cnt[0] <= cnt[1] ? 0 : 1;
+        cnt[(int)side[c]] += compSize[c];
+    }
+    for (int s = 0; s < n; ++s)
+        if (part[s] == 3) part[s] = side[comp[s]];
+    L = R = 0;
+    int S = 0;
+    for (int s = 0; s < n; ++s) (part[s] == 0 ? L : part[s] == 1 ? R : S) += 1;
+    if (S == 0) {
+        // a separator of at least one vertex keeps every separator node
+        // non-empty: the last vertex of the larger side
+        const char bigSide = L >= R ? 0 : 1;
+        for (int s = n - 1; s >= 0; --s)
+            if (part[s] == bigSide) {
+                part[s] = 2;
+                break;
+            }
+        (bigSide == 0 ? L : R) -= 1;
+        S = 1;
+    }
+    perm.assign(n, -1);
+    int o[3] = {0, L, L + R};
+    for (int s = 0; s < n; ++s) perm[s] = o[(int)part[s]]++;
+    return S;
+}
+
+void recurse_graph(const Graph& g, const std::vector<int>& perm, std::vector<PreNode>& list,
+                   int parent, int off, int cutoff) {
+    const int n = g.n;
+    if (n <= cutoff) {
+        PreNode node;
+        node.size = n;
+        node.off = off;
+        node.parent = parent;
+        node.inds = perm;
+        node.lowerStruct = connected_ancestors(g, nullptr, n, off);
+        list.push_back(std::move(node));
+        return;
+    }
+    int L, R;
+    std::vector<int> map;
+    const int sepSize = graph_bisect(g, L, R, map);
+    std::vector<int> inverseMap(n);
+    for (int s = 0; s < n; ++s) inverseMap[map[s]] = s;
+    const int me = (int)list.size();
+    {
+        PreNode node;
+        node.size = sepSize;
+        node.off = off + (n - sepSize);
+        node.parent = parent;
+        node.children.assign(2, -1);
+        std::vector<int> sepLocal(sepSize);
+        for (int s = 0; s < sepSize; ++s) sepLocal[s] = inverseMap[s + (n - sepSize)];
+        node.lowerStruct = connected_ancestors(g, sepLocal.data(), sepSize, off);
+        node.inds.resize(sepSize);
+        for (int s = 0; s < sepSize; ++s) node.inds[s] = perm[sepLocal[s]];
+        list.push_back(std::move(node));
+    }
+    Graph left, right;
+    build_children(g, map, inverseMap, L, left, R, right);
+    std::vector<int> leftPerm(L), rightPerm(R);
+    for (int s = 0; s < L; ++s) leftPerm[s] = perm[inverseMap[s]];
+    for (int s = 0; s < R; ++s) rightPerm[s] = perm[inverseMap[s + L]];
+    std::vector<int>().swap(inverseMap);
+    std::vector<int>().swap(map);
+    list[me].children[1] = (int)list.size();
+    {
+        Graph r = std::move(right);
+        recurse_graph(r, rightPerm, list, me, off + L, cutoff);
+    }
+    std::vector<int>().swap(rightPerm);
+    list[me].children[0] = (int)list.size();
+    recurse_graph(left, leftPerm, list, me, off, cutoff);
+}
+
 void recurse(int nx, int ny, int nz, const Graph& g, const std::vector<int>& perm,
              std::vector<PreNode>& list, int parent, int off, int cutoff) {
     const int n = g.n;
@@ -282,6 +475,10 @@ void analyze_postorder(Symbolic& sym) {
     }
 }
 
+namespace {
+void finish_dissection(std::vector<PreNode>& pre, const Graph& g, Symbolic& out);
+}
+
 void natural_nested_dissection(int nx, int ny, int nz, const Graph& g, int cutoff,
                                Symbolic& out) {
     if (nx <= 0 || ny <= 0 || nz <= 0 || (int64_t)nx * ny * nz != g.n)
@@ -291,7 +488,22 @@ void natural_nested_dissection(int nx, int ny, int nz, const Graph& g, int cutof
     std::vector<int> perm(g.n);
     std::iota(perm.begin(), perm.end(), 0);
     recurse(nx, ny, nz, g, perm, pre, -1, 0, cutoff);
+    finish_dissection(pre, g, out);
+}
 
+void graph_nested_dissection(const Graph& g, int cutoff, Symbolic& out) {
+    if (g.n <= 0) throw std::invalid_argument("empty graph");
+    if (cutoff < 1) throw std::invalid_argument("cutoff must be positive");
+    std::vector<PreNode> pre;
+    std::vector<int> perm(g.n);
+    std::iota(perm.begin(), perm.end(), 0);
+    recurse_graph(g, perm, pre, -1, 0, cutoff);
+    finish_dissection(pre, g, out);
+}
+
+namespace {
+// ReverseOrder, BuildMap and the symbolic analysis of a dissection's tree
+void finish_dissection(std::vector<PreNode>& pre, const Graph& g, Symbolic& out) {
     // ReverseOrder: node i of the creation order becomes node last-i.
     const int num = (int)pre.size();
     const int last = num - 1;
@@ -321,5 +533,6 @@ void natural_nested_dissection(int nx, int ny, int nz, const Graph& g, int cutof
     }
     analyze_postorder(out);
 }
+}  // namespace
 
 }  // namespace cliqg

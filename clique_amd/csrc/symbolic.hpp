@@ -60,6 +60,13 @@ Graph graph_from_csr(int n, const int64_t* rowOff, const int* cols);
 void natural_nested_dissection(int nx, int ny, int nz, const Graph& g,
                                int cutoff, Symbolic& out);
 
+// Nested dissection of a general graph (the reference's NestedDissection,
+// include/clique/symbolic/nested_dissection.hpp:491-533, with its recursion
+// :115-238) with this library's own vertex bisection in place of METIS
+// (symbolic.cpp graph_bisect): the same tree layout, numbering, map and
+// analysis; the separators themselves differ from METIS's.
+void graph_nested_dissection(const Graph& g, int cutoff, Symbolic& out);
+
 // Symbolic analysis for a user-provided separator tree given in the
 // reference's pre-reversal form is not needed at P=1; trees can instead be
 // supplied already in postorder through this entry (used for batching many

@@ -53,10 +53,14 @@ typedef enum { CLIQG_F64 = 0, CLIQG_F32 = 1 } cliqg_precision;
 typedef enum {
     CLIQG_FRONT_LDL = 0,          /* LDL_1D / LDL_2D: unit L, diagonal D            */
     CLIQG_FRONT_BLOCK_LDL = 1,    /* BLOCK_LDL_*: ATL := inverse, ABL original A21   */
-    CLIQG_FRONT_LDL_INTRAPIV = 2  /* LDL_INTRAPIV_*: Bunch-Kaufman pivoting inside
+    CLIQG_FRONT_LDL_INTRAPIV = 2, /* LDL_INTRAPIV_*: Bunch-Kaufman pivoting inside
                                      each front's ATL (FrontLDLIntraPiv,
                                      ldl/local_front.hpp:73-94): unit L, D with
                                      1x1 and 2x2 blocks, a permutation per front  */
+    CLIQG_FRONT_BLOCK_LDL_INTRAPIV = 3 /* BLOCK_LDL_INTRAPIV_*: FrontBlockLDL with
+                                     intra-front pivoting (ldl/local_front_block.hpp:
+                                     35-57): ATL := inverse of the front's ATL,
+                                     ABL original A21; no pivots are kept          */
 } cliqg_front_type;
 
 /* which-array selectors for cliqg_symbolic_node_array */
@@ -98,6 +102,17 @@ cliqg_status cliqg_device_synchronize(void);
 cliqg_status cliqg_symbolic_natural(int nx, int ny, int nz, int N,
                                     const int64_t* h_row_off, const int* h_cols,
                                     int cutoff, cliqg_symbolic* out);
+/* Nested dissection of a general graph: replaces NestedDissection
+ * (include/clique/symbolic/nested_dissection.hpp:35-44, 491-533) at world
+ * size 1. The recursion, node numbering, map and analysis are the
+ * reference's; each bisection is this library's own (a breadth-first level
+ * structure from a pseudo-peripheral vertex, its best-balancing level as the
+ * separator, separator vertices with neighbours on one side only moved to
+ * that side) instead of METIS's, so separators differ from the reference's
+ * METIS orderings. The graph is A's CSR pattern (symmetric), as for
+ * cliqg_symbolic_natural. */
+cliqg_status cliqg_symbolic_graph(int N, const int64_t* h_row_off, const int* h_cols, int cutoff,
+                                  cliqg_symbolic* out);
 /* A forest of independent problems: symbolic i occupies nodal and original
  * indices shifted by the sizes of symbolics 0..i-1 (batched solves of many
  * unrelated systems in one schedule; no reference counterpart). */

@@ -589,29 +589,21 @@ inline std::vector<int> node_array(cliqg_symbolic h, int s, int which, int n) {
 }
 }  // namespace detail
 
-// Natural (geometric) nested dissection of an nx x ny x nz grid graph
-// (include/clique/symbolic/natural_nested_dissection.hpp:17-26, 447-488).
-inline void NaturalNestedDissection(int nx, int ny, int nz, const DistGraph& graph,
-                                    DistMap& map, DistSeparatorTree& sepTree,
-                                    DistSymmInfo& info, int cutoff = 128,
-                                    bool storeFactRecvInds = false) {
-    (void)storeFactRecvInds;
-    const int N = graph.NumSources();
-    cliqg_symbolic h = nullptr;
-    detail::check(cliqg_symbolic_natural(nx, ny, nz, N, graph.Offsets().data(),
-                                         graph.Targets().data(), cutoff, &h),
-                  "NaturalNestedDissection");
+namespace detail {
+// map, sepTree and info of a finished symbolic analysis h (N vertices)
+inline void take_symbolic(cliqg_symbolic h, int N, DistMap& map, DistSeparatorTree& sepTree,
+                          DistSymmInfo& info, const char* what) {
     info.symbolic.reset(h, detail::SymbolicDeleter());
     int num = 0;
-    detail::check(cliqg_symbolic_num_nodes(h, &num, nullptr), "NaturalNestedDissection");
+    detail::check(cliqg_symbolic_num_nodes(h, &num, nullptr), what);
     map.Resize(N);
-    detail::check(cliqg_symbolic_map(h, map.Map().data()), "NaturalNestedDissection");
+    detail::check(cliqg_symbolic_map(h, map.Map().data()), what);
     sepTree.Clear();
     sepTree.distSeps.clear();
     info.localNodes.assign(num, SymmNodeInfo());
     for (int s = 0; s < num; ++s) {
         int in[8];
-        detail::check(cliqg_symbolic_node(h, s, in), "NaturalNestedDissection");
+        detail::check(cliqg_symbolic_node(h, s, in), what);
         SymmNodeInfo& nd = info.localNodes[s];
         nd.size = in[0];
         nd.off = in[1];
@@ -655,14 +647,42 @@ inline void NaturalNestedDissection(int nx, int ny, int nz, const DistGraph& gra
         d.leftSize = d.rightSize = -1;
     }
 }
+}  // namespace detail
 
-// METIS-based dissection (symbolic/nested_dissection.hpp:35-44): METIS may
-// not be redistributed, so this build offers the geometric dissection only.
-inline void NestedDissection(const DistGraph&, DistMap&, DistSeparatorTree&, DistSymmInfo&,
-                             bool = true, int = 1, int = 1, int = 128, bool = false) {
-    throw LogicError(
-        "METIS-based NestedDissection is not available in this build; "
-        "use NaturalNestedDissection with the grid dimensions");
+// Natural (geometric) nested dissection of an nx x ny x nz grid graph
+// (include/clique/symbolic/natural_nested_dissection.hpp:17-26, 447-488).
+inline void NaturalNestedDissection(int nx, int ny, int nz, const DistGraph& graph,
+                                    DistMap& map, DistSeparatorTree& sepTree,
+                                    DistSymmInfo& info, int cutoff = 128,
+                                    bool storeFactRecvInds = false) {
+    (void)storeFactRecvInds;
+    const int N = graph.NumSources();
+    cliqg_symbolic h = nullptr;
+    detail::check(cliqg_symbolic_natural(nx, ny, nz, N, graph.Offsets().data(),
+                                         graph.Targets().data(), cutoff, &h),
+                  "NaturalNestedDissection");
+    detail::take_symbolic(h, N, map, sepTree, info, "NaturalNestedDissection");
+}
+
+// Nested dissection of a general graph (symbolic/nested_dissection.hpp:35-44,
+// 491-533): the reference's recursion, numbering, map and analysis, with the
+// library's own vertex bisection in place of METIS (which may not be
+// redistributed), so the separators differ from the reference's. The
+// distribution arguments have no effect at world size 1.
+inline void NestedDissection(const DistGraph& graph, DistMap& map, DistSeparatorTree& sepTree,
+                             DistSymmInfo& info, bool sequential = true, int numDistSeps = 1,
+                             int numSeqSeps = 1, int cutoff = 128,
+                             bool storeFactRecvInds = false) {
+    (void)sequential;
+    (void)numDistSeps;
+    (void)numSeqSeps;
+    (void)storeFactRecvInds;
+    const int N = graph.NumSources();
+    cliqg_symbolic h = nullptr;
+    detail::check(cliqg_symbolic_graph(N, graph.Offsets().data(), graph.Targets().data(), cutoff,
+                                       &h),
+                  "NestedDissection");
+    detail::take_symbolic(h, N, map, sepTree, info, "NestedDissection");
 }
 
 // ---------------------------------------------------------------------------
@@ -823,7 +843,7 @@ public:
                           "MirrorFronts");
             // pivoted fronts also keep D's subdiagonal and the permutation
             // (ldl/local.hpp:108-115)
-            if (PivotedFactorization(frontType)) {
+            if (PivotedFactorization(frontType) && !BlockFactorization(frontType)) {
                 localFronts[s].subdiag.Resize(n, 1);
                 localFronts[s].piv.Resize(n, 1);
                 detail::check(cliqg_plan_get_front_pivots(plan_.get(), s,
@@ -942,12 +962,10 @@ inline void LDL(DistSymmInfo& info, DistSymmFrontTree<F>& L, SymmFrontType newTy
     if (!Unfactored(L.frontType)) throw LogicError("Matrix is already factored");
     ChangeFrontType(L, SYMM_2D);
     const SymmFrontType initial = InitialFactorType(newType);
-    if (BlockFactorization(initial) && PivotedFactorization(initial))
-        throw LogicError("Block LDL with intra-front pivoting is not available in this build");
     if (L.hostFronts_) L.PushFronts();
-    const cliqg_front_type ft = BlockFactorization(initial)     ? CLIQG_FRONT_BLOCK_LDL
-                                : PivotedFactorization(initial) ? CLIQG_FRONT_LDL_INTRAPIV
-                                                                : CLIQG_FRONT_LDL;
+    const bool blk = BlockFactorization(initial), piv = PivotedFactorization(initial);
+    const cliqg_front_type ft = blk ? (piv ? CLIQG_FRONT_BLOCK_LDL_INTRAPIV : CLIQG_FRONT_BLOCK_LDL)
+                                    : (piv ? CLIQG_FRONT_LDL_INTRAPIV : CLIQG_FRONT_LDL);
     detail::check(cliqg_plan_factor(L.Plan(), L.DeviceValues(),
                                     precision == FACTOR_FP32 ? CLIQG_F32 : CLIQG_F64, ft),
                   "LDL");
@@ -1104,10 +1122,35 @@ inline void Multiply(T alpha, const DistSparseMatrix<T>& A, const DistMultiVec<T
                   "Multiply");
 }
 
-// Black-box solve (numeric/solve.hpp:94-116) on a grid graph: natural
-// dissection of the nx x ny x nz grid, LDL with intra-front pivoting
-// (LDL_INTRAPIV_1D, the reference's choice at numeric/solve.hpp:110) and Solve. The reference's
-// METIS-based overload without grid dimensions is not available.
+// The reference's black-box solve (numeric/solve.hpp:94-116): nested
+// dissection of A's graph, LDL with intra-front pivoting (LDL_INTRAPIV_1D,
+// numeric/solve.hpp:110), Pull, Solve, Push.
+template <typename F>
+inline void SymmetricSolve(const DistSparseMatrix<F>& A, DistMultiVec<F>& X, bool conjugate = false,
+                           bool sequential = true, int numDistSeps = 1, int numSeqSeps = 1,
+                           int cutoff = 128) {
+    DistSymmInfo info;
+    DistSeparatorTree sepTree;
+    DistMap map, inverseMap;
+    NestedDissection(A.LockedDistGraph(), map, sepTree, info, sequential, numDistSeps, numSeqSeps,
+                     cutoff);
+    map.FormInverse(inverseMap);
+    DistSymmFrontTree<F> frontTree(A, map, sepTree, info, conjugate);
+    LDL(info, frontTree, LDL_INTRAPIV_1D);
+    DistNodalMultiVec<F> XNodal;
+    XNodal.Pull(inverseMap, info, X);
+    Solve(info, frontTree, XNodal);
+    XNodal.Push(inverseMap, info, X);
+}
+template <typename F>
+inline void HermitianSolve(const DistSparseMatrix<F>& A, DistMultiVec<F>& X, bool sequential = true,
+                           int numDistSeps = 1, int numSeqSeps = 1, int cutoff = 128) {
+    SymmetricSolve(A, X, true, sequential, numDistSeps, numSeqSeps, cutoff);
+}
+
+// The same on a grid graph: natural dissection of the nx x ny x nz grid
+// (the geometric ordering the reference's tests use), LDL (intra-front
+// pivoting unless frontType says otherwise) and Solve.
 template <typename F>
 inline void SymmetricSolve(const DistSparseMatrix<F>& A, DistMultiVec<F>& X, int nx, int ny,
                            int nz, bool conjugate = false, int cutoff = 128,

@@ -188,9 +188,10 @@ class Factor:
             raise RuntimeError(f"or_initialize failed ({r})")
 
     def ldl(self, nb=96, block=False, nthreads=0, pivoted=False):
-        """LocalLDL: FrontLDL (default), FrontBlockLDL (block) or
-        FrontLDLIntraPiv with Bunch-Kaufman pivoting (pivoted)."""
-        mode = 2 if pivoted else int(bool(block))
+        """LocalLDL: FrontLDL (default), FrontBlockLDL (block),
+        FrontLDLIntraPiv with Bunch-Kaufman pivoting (pivoted) or
+        FrontBlockLDL with intra-front pivoting (both)."""
+        mode = (3 if block else 2) if pivoted else int(bool(block))
         r = lib().or_ldl(self._h, nb, mode, nthreads)
         if r != 0:
             raise ZeroDivisionError(f"oracle LDL hit a zero pivot ({r})")

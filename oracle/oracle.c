@@ -464,8 +464,10 @@ int or_ldl(or_factor* f, int nb, int block, int nthreads) {
     (void)nthreads;
 #endif
     if (nb < 1) return -1;
-    f->block = block == 1;
-    f->pivoted = block == 2;
+    /* 0: FrontLDL, 1: FrontBlockLDL, 2: FrontLDLIntraPiv, 3: FrontBlockLDL
+     * with intra-front pivoting */
+    f->block = block == 1 || block == 3;
+    f->pivoted = block == 2 || block == 3;
     int r = f->is_complex ? ldl_z(f, nb, f->block) : ldl_d(f, nb, f->block);
     if (r == 0) f->factored = 1;
     return r;

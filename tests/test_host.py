@@ -125,3 +125,103 @@ def test_relative_indices_increase(dims, cut):
                 own = rel < nd["size"]
                 # a prefix of own-column indices, then the update rows
                 assert not np.any(own[1:] & ~own[:-1]), (k, w)
+
+
+# ---------------------------------------------------------------------------
+# nested dissection of general graphs (NestedDissection with the library's
+# own bisection)
+# ---------------------------------------------------------------------------
+def irregular_graph(seed, n=600, deg=6):
+    """A random geometric graph in the unit square (k nearest neighbours,
+    symmetrised) plus a second, disconnected component: irregular degrees,
+    no grid structure."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(size=(n, 2))
+    d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    nbr = np.argsort(d2, axis=1)[:, 1:deg + 1]
+    rows = np.repeat(np.arange(n), deg)
+    A = sp.csr_matrix((np.ones(rows.size), (rows, nbr.ravel())), shape=(n, n))
+    A = A + A.T
+    ip2, ind2, _, _ = P.stencil7(7, 6, 1, 4.0)
+    B = sp.csr_matrix((np.ones(ind2.size), ind2, ip2))
+    M = (sp.block_diag([A, B]) + sp.identity(n + B.shape[0])).tocsr()
+    M.has_sorted_indices = False
+    M.sort_indices()
+    return M.indptr.astype(np.int64), M.indices.astype(np.int32)
+
+
+def dissection_checks(ip, ind, cut):
+    s = cq.NestedDissection(ip, ind, cutoff=cut)
+    N = ip.size - 1
+    mp = s.map()
+    assert np.array_equal(np.sort(mp), np.arange(N))
+    nodes = [s.node(k) for k in range(s.num_nodes)]
+    assert nodes[-1]["parent"] == -1
+    # subtree nodal ranges: a node's subtree owns [first, off + size)
+    first = {}
+    for k, nd in enumerate(nodes):   # postorder: children come first
+        kids = [c for c in (nd["child0"], nd["child1"]) if c >= 0]
+        assert len(kids) in (0, 2)
+        first[k] = min([first[c] for c in kids] + [nd["off"]])
+        if not kids:
+            assert 1 <= nd["size"] <= cut
+        else:
+            assert nd["size"] >= 1
+            assert nodes[kids[0]]["parent"] == k and nodes[kids[1]]["parent"] == k
+    # separator property: no edge joins the two children's subtrees
+    rows = np.repeat(np.arange(N), np.diff(ip))
+    a, b = mp[rows], mp[ind]
+    for k, nd in enumerate(nodes):
+        if nd["child0"] < 0:
+            continue
+        l, r = nd["child0"], nd["child1"]
+        inL = lambda x: (x >= first[l]) & (x < nodes[l]["off"] + nodes[l]["size"])   # noqa: E731
+        inR = lambda x: (x >= first[r]) & (x < nodes[r]["off"] + nodes[r]["size"])   # noqa: E731
+        assert not np.any(inL(a) & inR(b))
+    return s
+
+
+@pytest.mark.parametrize("seed,cut", [(0, 16), (1, 40), (2, 128)])
+def test_graph_dissection_separates_an_irregular_graph(seed, cut):
+    ip, ind = irregular_graph(seed)
+    dissection_checks(ip, ind, cut)
+
+
+@pytest.mark.parametrize("dims,cut", [((6, 5, 4), 6), ((9, 8, 1), 5), ((30, 1, 1), 3)])
+def test_graph_dissection_lower_struct_holds_symbolic_cholesky(dims, cut):
+    """The analysis of a dissection built by the graph bisection holds the
+    column structure of the Cholesky factor of P A P^T (brute force): every
+    fill entry below a node lies in its lowerStruct. Equality is not implied:
+    the supernodal rule (lowerStruct = children's structures and the node's
+    own connections, less the node, LocalSymmetricAnalysis.cpp:15-114)
+    treats a separator as one dense block, and a separator of a general
+    graph need not be connected (METIS's need not either)."""
+    from test_oracle import symbolic_cholesky
+    ip, ind, v, d = P.stencil7(*dims, 6.0)
+    s = dissection_checks(ip, ind, cut)
+    A = P.to_scipy(ip, ind, v).toarray()
+    mp = s.map()
+    Ap = np.zeros_like(A)
+    Ap[np.ix_(mp, mp)] = A
+    Lpat = symbolic_cholesky(Ap)
+    for k in range(s.num_nodes):
+        nd = s.node(k)
+        cols = range(nd["off"], nd["off"] + nd["size"])
+        below = set()
+        for c in cols:
+            below |= set(np.nonzero(Lpat[:, c])[0].tolist())
+        below -= set(cols)
+        assert below <= set(s.node_array(k, 1).tolist())
+
+
+def test_graph_dissection_fill_is_close_to_natural_on_grids():
+    """On grids the graph bisection's factor is within a small factor of the
+    natural (geometric) dissection's in entries and flops."""
+    for dims in ((24, 24, 24), (120, 90, 1)):
+        ip, ind, v, d = P.stencil7(*dims, 6.0)
+        g = cq.NestedDissection(ip, ind, cutoff=64)
+        n = cq.NaturalNestedDissection(*d, ip, ind, cutoff=64)
+        fg, _, eg = g.work(False)
+        fn, _, en = n.work(False)
+        assert eg <= 1.6 * en and fg <= 2.5 * fn, (dims, eg / en, fg / fn)
