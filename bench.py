@@ -430,7 +430,8 @@ def main():
     g = prof32[upd_name]
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12
     kernel_of = {"gemm_update": "k_gemm<cfloat,64,64,3>", "gemm_update_w8": "k_gemm<cfloat,128,128,3,16,2,4>",
-                 "gemm_update_small": "k_gemm<cfloat,32,32,3>", "gemm_update_large": "k_gemm<float,128,128,3>"}
+                 "gemm_update_small": "k_gemm<cfloat,32,32,3>", "gemm_update_large": "k_gemm<float,128,128,3>",
+                 "gemm_update_k64": "k_gemm<cfloat,32,32,3,64>"}
     # complex FP32 products issue three real MFMAs per complex block (Gauss)
     # where the flop convention counts four real multiplications
     mfma_share = 0.75 if cplx else 1.0

@@ -349,7 +349,8 @@ class DistSymmFrontTree:
         return v.value
 
 
-_SHAPES = {0: "64x64", 4: "128x128", 5: "32x32", 6: "128x128x8w"}
+_SHAPES = {0: "64x64", 4: "128x128", 5: "32x32", 6: "128x128x8w", 7: "32x32k64"}
+_SHAPE_ORDER = ("32x32", "64x64", "128x128", "128x128x8w", "32x32k64")   # GEMM_VARIANTS
 
 
 def autotune_table():
@@ -362,8 +363,7 @@ def autotune_table():
     for line in buf.value.decode().splitlines():
         key, v, *ms = line.split()
         out[key] = {"shape": _SHAPES.get(int(v), v),
-                    "ms": {name: float(m) for name, m in zip(("32x32", "64x64", "128x128",
-                                                              "128x128x8w"), ms)
+                    "ms": {name: float(m) for name, m in zip(_SHAPE_ORDER, ms)
                            if float(m) >= 0}}
     return out
 

@@ -1,7 +1,6 @@
 // Device kernels of libcliqg for gfx950 (CDNA4). See kernels.hpp.
 #include "kernels.hpp"
 
-#include <cstdlib>
 #include <type_traits>
 
 namespace cliqg {
@@ -166,7 +165,9 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 }  // namespace
 
 int gemm_tile_m(int variant) {
-    return variant == TILE_32 ? 32 : ((variant == TILE_128 || variant == TILE_128_8W) ? 128 : 64);
+    return (variant == TILE_32 || variant == TILE_32_K64)
+               ? 32
+               : ((variant == TILE_128 || variant == TILE_128_8W) ? 128 : 64);
 }
 int gemm_tile_n(int variant) { return gemm_tile_m(variant); }
 
@@ -524,6 +525,11 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
             throw Error(1, "eight-wave 128x128 GEMM tiles are built for complex FP32 only");
     } else if (variant == TILE_32) {
         launch_gemm_shape<T, 32, 32>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+    } else if (variant == TILE_32_K64) {
+        if constexpr (gemm_variant_allowed<T>(TILE_32_K64))
+            launch_gemm_shape<T, 32, 32, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
+        else
+            throw Error(1, "64-deep 32x32 GEMM tiles are not built for complex FP64");
     } else {
         launch_gemm_shape<T, 64, 64>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     }
@@ -839,215 +845,10 @@ __attribute__((amdgpu_waves_per_eu(1, Scalar<T>::complex ? 4 : 8))) void k_diag_
         }
 }
 
-// ---------------------------------------------------------------------------
-// Diagonal block, blocked: the same outputs as k_diag_ldl (unit L11 into the
-// front, the pivots in FP64 field type, the inverse of the unit lower
-// factor) from an LDL^T in four 16-wide sub-panels, all in FP64 (field type)
-// in LDS. Per sub-panel: wave 0 factors the 16 x 16 diagonal sub-block
-// (sixteen steps inside one wave, no workgroup barrier) and inverts its unit
-// lower factor; all waves then form the unscaled TRSM rows below it,
-// W = A21 Ls^{-T}, and L21 = W D^{-1}, and apply the rank-16 update to the
-// rest of the block. The inverse of the whole unit factor follows by block
-// substitution, X_rq = -X_rr sum_p L_rp X_pq, one block diagonal at a time.
-// About twenty barriers per block instead of sixty-four. Rows and columns
-// at or past b are padded with the identity (they decouple, and nothing of
-// them is written back but the zeros of Linv).
-// LDS layout: M column-major (ld NB + 1): the lower triangle holds the block
-// being factored, the strict upper triangle the transposed inverse
-// (X(i, j), i > j, at M[j + i * LD]).
-// ---------------------------------------------------------------------------
-template <class T, bool HERM>
-__global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(1, Scalar<T>::complex ? 4 : 8))) void k_diag_ldl_blk(
-    const DiagDesc<T>* __restrict__ descs, int* status) {
-    using W = typename Scalar<T>::wide;
-    constexpr int LD = NB + 1, SP = 16, LW = NB + 1;
-    __shared__ W M[NB * LD];
-    __shared__ W Wb[SP * LW];   // W(r, c) at Wb[r + c * LW]; block products T in phase D
-    __shared__ W dv[NB];
-
-    __builtin_amdgcn_s_setprio(3);
-    const DiagDesc<T> d = descs[blockIdx.x];
-    const int b = d.b, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    auto at = [&](int i, int j) -> W& { return M[i + j * LD]; };
-    auto op = [](W x) { return HERM ? conj_(x) : x; };
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e % NB, j = e / NB;
-        W v{};
-        if (i >= j) {
-            if (i < b && j < b) v = cvt<W>(d.A11[i + (int64_t)j * d.lda]);
-            else if (i == j) v = one<W>();
-        }
-        at(i, j) = v;
-    }
-    __syncthreads();
-    for (int c0 = 0; c0 < NB; c0 += SP) {
-        if (c0 >= b) break;   // the rest is identity padding
-        // (A) wave 0: the diagonal sub-block, then its unit lower inverse
-        if (wave == 0) {
-            const int i = lane & 15, cg = lane >> 4;
-            for (int k = 0; k < SP; ++k) {
-                W dk = at(c0 + k, c0 + k);
-                if (HERM) dk = real_part_only(dk);
-                const W rdk = pivot_inverse(dk);
-                const W wi = at(c0 + i, c0 + k);
-                W wj[4];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) wj[t] = at(c0 + cg + 4 * t, c0 + k);
-                __builtin_amdgcn_wave_barrier();
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int j = cg + 4 * t;
-                    if (j > k && i >= j) at(c0 + i, c0 + j) -= wi * rdk * op(wj[t]);
-                }
-                if (cg == 0 && i > k) at(c0 + i, c0 + k) = wi * rdk;
-                if (lane == 0) dv[c0 + k] = dk;
-                __builtin_amdgcn_wave_barrier();
-            }
-            // X = Ls^{-1}: row i from rows p < i (lane j < 16 owns column j)
-            for (int r = 1; r < SP; ++r) {
-                if (lane < r) {
-                    const int j = lane;
-                    W acc = at(c0 + r, c0 + j);   // L(r, j) X(j, j), X(j, j) = 1
-                    for (int p = j + 1; p < r; ++p) acc += at(c0 + r, c0 + p) * at(c0 + j, c0 + p);
-                    at(c0 + j, c0 + r) = -acc;
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-        __syncthreads();
-        // (B) rows below the sub-block: W = A21 Ls^{-T} (^{-H}), L21 = W D^{-1}
-        const int r0 = c0 + SP, nr = NB - r0;
-        W wv[3];
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            wv[t] = W{};
-            if (e < nr * SP) {
-                const int r = r0 + e / SP, c = e % SP;
-                W acc = at(r, c0 + c);
-                for (int p = 0; p < c; ++p) acc += at(r, c0 + p) * op(at(c0 + p, c0 + c));
-                wv[t] = acc;
-            }
-        }
-        __syncthreads();
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            if (e < nr * SP) {
-                const int r = r0 + e / SP, c = e % SP;
-                Wb[r - r0 + c * LW] = wv[t];
-                at(r, c0 + c) = wv[t] * pivot_inverse(dv[c0 + c]);
-            }
-        }
-        __syncthreads();
-        // (C) rank-16 update of the rest: A(r, s) -= sum_c W(r, c) op(L(s, c))
-        const int ne = nr * (nr + 1) / 2;
-        for (int e = tid; e < ne; e += 256) {
-            // e -> (r, s) with s <= r, column-major over the lower triangle of order nr
-            int s = (int)((2.0 * nr + 1.0 - sqrt((2.0 * nr + 1.0) * (2.0 * nr + 1.0) - 8.0 * e)) * 0.5);
-            while (s > 0 && s * (2 * nr - s + 1) / 2 > e) --s;
-            while ((s + 1) * (2 * nr - s) / 2 <= e) ++s;
-            const int r = s + (e - s * (2 * nr - s + 1) / 2);
-            W acc = at(r0 + r, r0 + s);
-#pragma unroll 4
-            for (int c = 0; c < SP; ++c) acc -= Wb[r + c * LW] * op(at(r0 + s, c0 + c));
-            at(r0 + r, r0 + s) = acc;
-        }
-        __syncthreads();
-    }
-    // (D) off-diagonal blocks of X = L^{-1}, one block diagonal at a time:
-    // T_rq = sum_{q <= p < r} L_rp X_pq, then X_rq = -X_rr T_rq
-    const int nb16 = (b + SP - 1) / SP;
-    for (int dist = 1; dist < nb16; ++dist) {
-        const int nblk = nb16 - dist;
-        W tv[3];
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            tv[t] = W{};
-            if (e < nblk * SP * SP) {
-                const int q = e / (SP * SP), i = (e / SP) % SP, j = e % SP;
-                const int rr = q + dist;
-                W acc{};
-                for (int p = q; p < rr; ++p)
-                    for (int l = 0; l < SP; ++l) {
-                        // L(rr, p)(i, l) X(p, q)(l, j); X(p, p) is the
-                        // sub-inverse (unit diagonal), X(p, q) for p > q stored
-                        // transposed in the upper triangle
-                        const int gi = rr * SP + i, gl = p * SP + l, gj = q * SP + j;
-                        const W x = (gl == gj) ? one<W>() : (gl > gj ? at(gj, gl) : W{});
-                        acc += at(gi, gl) * x;
-                    }
-                tv[t] = acc;
-            }
-        }
-        __syncthreads();
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            if (e < nblk * SP * SP) Wb[e] = tv[t];
-        }
-        __syncthreads();
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            if (e < nblk * SP * SP) {
-                const int q = e / (SP * SP), i = (e / SP) % SP, j = e % SP;
-                const int rr = q + dist;
-                // X_rq(i, j) = -sum_l X_rr(i, l) T(l, j)
-                W acc{};
-                for (int l = 0; l <= i; ++l) {
-                    const W x = (l == i) ? one<W>() : at(rr * SP + l, rr * SP + i);
-                    acc += x * Wb[q * SP * SP + l * SP + j];
-                }
-                tv[t] = -acc;
-            }
-        }
-        __syncthreads();
-        for (int t = 0; t < 3; ++t) {
-            const int e = tid + 256 * t;
-            if (e < nblk * SP * SP) {
-                const int q = e / (SP * SP), i = (e / SP) % SP, j = e % SP;
-                const int rr = q + dist;
-                at(q * SP + j, rr * SP + i) = tv[t];
-            }
-        }
-        __syncthreads();
-    }
-    // write-out
-    W* Dout = reinterpret_cast<W*>(d.D);
-    for (int j = tid; j < b; j += 256) {
-        Dout[j] = dv[j];
-        if (is_zero_or_nonfinite(dv[j])) record_status(status, 2, d.front, d.col0 + j);
-    }
-    for (int e = tid; e < NB * NB; e += 256) {
-        const int i = e % NB, j = e / NB;
-        if (i < b && j < b && i >= j)
-            d.A11[i + (int64_t)j * d.lda] = (i == j) ? one<T>() : cvt<T>(at(i, j));
-        T x = zero<T>();
-        if (i < b && j < b) x = (i == j) ? one<T>() : (i > j ? cvt<T>(at(j, i)) : zero<T>());
-        d.Linv[i + j * NB] = x;
-    }
-}
-
-// CLIQG_DIAG_BLOCKED=0 selects the register-cyclic kernel (k_diag_ldl)
-static bool diag_blocked() {
-    static const bool v = [] {
-        const char* e = std::getenv("CLIQG_DIAG_BLOCKED");
-        return !(e && std::atoi(e) == 0);
-    }();
-    return v;
-}
-
 template <class T>
 void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
                      hipStream_t s) {
     if (ndesc <= 0) return;
-    if (diag_blocked()) {
-        if (herm)
-            hipLaunchKernelGGL((k_diag_ldl_blk<T, true>), dim3(ndesc), dim3(256), 0, s, d_descs,
-                               d_status);
-        else
-            hipLaunchKernelGGL((k_diag_ldl_blk<T, false>), dim3(ndesc), dim3(256), 0, s, d_descs,
-                               d_status);
-        return;
-    }
     if (herm)
         hipLaunchKernelGGL((k_diag_ldl<T, true>), dim3(ndesc), dim3(256), 0, s, d_descs, d_status);
     else

@@ -17,6 +17,8 @@ enum GemmTile : int {
     TILE_128 = 4,     // 128x128 tiles, four waves of 64x64 (real types, large launches)
     TILE_32 = 5,      // 32x32 tiles, four waves of 16x16 (latency-bound launches)
     TILE_128_8W = 6,  // 128x128 tiles, eight waves of 64x32 (complex FP32)
+    TILE_32_K64 = 7,  // 32x32 tiles with 64-deep K chunks (latency-bound launches
+                      // with small K: one global round trip per 64 of K, not per 16)
 };
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
@@ -29,10 +31,14 @@ template <class T>
 constexpr bool gemm_variant_allowed(int variant) {
     if (variant == TILE_128) return !Scalar<T>::complex;
     if (variant == TILE_128_8W) return Scalar<T>::complex && sizeof(T) == 8;
+    // 64-deep chunks of complex FP64 would need 192 KB of LDS
+    if (variant == TILE_32_K64) return !(Scalar<T>::complex && sizeof(T) == 16);
     return variant == TILE_64 || variant == TILE_32;
 }
 // Every tile shape, in the order the autotuner tries them
-constexpr int GEMM_VARIANTS[4] = {TILE_32, TILE_64, TILE_128, TILE_128_8W};
+constexpr int NUM_GEMM_VARIANTS = 5;
+constexpr int GEMM_VARIANTS[NUM_GEMM_VARIANTS] = {TILE_32, TILE_64, TILE_128, TILE_128_8W,
+                                                  TILE_32_K64};
 
 enum GemmFlags : int {
     G_TRANSA = 1,   // op(A)(i,k) = A[k + i*lda]   (else A[i + k*lda])

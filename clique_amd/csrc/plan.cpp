@@ -159,9 +159,9 @@ struct Step {
     // GEMM tile shapes the launch may take (the autotuner picks among them):
     // shape, arena offset of its tile-start array and its tile count
     int nvar = 0;
-    int vars[4] = {0, 0, 0, 0};
-    size_t vstart[4] = {0, 0, 0, 0};
-    int vtiles[4] = {0, 0, 0, 0};
+    int vars[NUM_GEMM_VARIANTS] = {};
+    size_t vstart[NUM_GEMM_VARIANTS] = {};
+    int vtiles[NUM_GEMM_VARIANTS] = {};
     int bucket = -1;      // the launch's autotuner bucket (index into the tuning table)
     bool scaled = false;  // GEMM descriptors carry pivots (scaleK)
     bool update = false;  // a factor update (profiled as gemm_update, role 3 kernels)
@@ -187,7 +187,7 @@ struct TuneTable {
     std::vector<int> choice;   // tuned shape per bucket, -1 if none
     // milliseconds per tile shape (GEMM_VARIANTS order) summed over the
     // bucket's launches in the last tuning pass, -1 where not measured
-    std::vector<std::array<double, 4>> ms;
+    std::vector<std::array<double, NUM_GEMM_VARIANTS>> ms;
     bool loaded = false;
 };
 TuneTable& tune_table() {
@@ -205,7 +205,9 @@ int tune_bucket_locked(TuneTable& t, const std::string& key) {
     t.index.emplace(key, b);
     t.keys.push_back(key);
     t.choice.push_back(-1);
-    t.ms.push_back({-1.0, -1.0, -1.0, -1.0});
+    std::array<double, NUM_GEMM_VARIANTS> none;
+    none.fill(-1.0);
+    t.ms.push_back(none);
     return b;
 }
 void tune_load_locked(TuneTable& t) {
@@ -235,7 +237,8 @@ int tune_bucket(int type, int role, bool lower, double tiles64, double meanK) {
     return tune_bucket_locked(t, key);
 }
 // CLIQG_GEMM_TILE=<shape>: every launch that admits the shape takes it
-// (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave), ahead of the
+// (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave, 7: 32x32 with
+// 64-deep K chunks), ahead of the
 // tuned and the modelled choice
 int forced_variant() {
     static const int v = [] {
@@ -263,7 +266,7 @@ void tune_save_locked(TuneTable& t) {
     std::fclose(f);
 }
 int variant_slot(int v) {
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NUM_GEMM_VARIANTS; ++i)
         if (GEMM_VARIANTS[i] == v) return i;
     return -1;
 }
@@ -429,6 +432,13 @@ struct StepBuilder {
             bool ok = true;
             for (const auto& d : ds)
                 if ((d.C == d.A || d.C == d.B) && d.N > gemm_tile_n(v)) ok = false;
+            // a triangular K range (G_KLO, G_KHI) is widened to whole
+            // chunks: 64-deep chunks read up to 63 entries of op(A) past
+            // the triangle, which only the solves' inverted blocks hold
+            // as zeros
+            if (v == TILE_32_K64 && !solve)
+                for (const auto& d : ds)
+                    if (d.flags & (G_KLO | G_KHI)) ok = false;
             if (!ok) continue;
             std::vector<int> start;
             int total = 0;
@@ -1295,6 +1305,7 @@ struct Numeric : NumericBase {
             const bool bwd = pass == 1;
             std::vector<Step>& steps = bwd ? S.bwd : S.fwd;
             StepBuilder<T> sb{S.arena, steps};
+            sb.solve = true;
             for (int hh = 0; hh < P.numLevels; ++hh) {
                 const int h = bwd ? P.numLevels - 1 - hh : hh;
                 const auto& lv = P.levels[h];
@@ -1607,12 +1618,13 @@ struct Numeric : NumericBase {
                 const int kind = (s.kind == S_GEMM && S) ? 7 : s.kind;
                 // factor GEMMs by instantiation: pivot-scaled updates (Schur,
                 // panel and outer-panel updates) apart from the TRSM-like rest
-                // (32x32- and 128x128-tile updates apart: different kernels)
+                // (each tile shape's updates apart: different kernels)
                 const char* name = (s.kind == S_GEMM && !S)
                                        ? ((s.update || s.scaled)
                                               ? (s.variant == TILE_32       ? "gemm_update_small"
                                                  : s.variant == TILE_128    ? "gemm_update_large"
                                                  : s.variant == TILE_128_8W ? "gemm_update_w8"
+                                                 : s.variant == TILE_32_K64 ? "gemm_update_k64"
                                                                             : "gemm_update")
                                               : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
@@ -1788,33 +1800,37 @@ struct Numeric : NumericBase {
         tuning = nullptr;
         CQ_HIP(hipStreamSynchronize(st));
         // per launch: milliseconds of each tile shape
-        std::map<int, std::pair<int, std::array<double, 4>>> perStep;   // step -> (bucket, ms)
+        constexpr int NV = NUM_GEMM_VARIANTS;
+        using Times = std::array<double, NV>;
+        Times unmeasured;
+        unmeasured.fill(-1.0);
+        std::map<int, std::pair<int, Times>> perStep;   // step -> (bucket, ms)
         for (const TuneSample& x : samples) {
             float ms = 0.f;
             CQ_HIP(hipEventElapsedTime(&ms, x.e0, x.e1));
             auto it = perStep.find(x.step);
             if (it == perStep.end())
-                it = perStep.emplace(x.step, std::make_pair(x.bucket, std::array<double, 4>{
-                                                                          -1.0, -1.0, -1.0, -1.0}))
-                         .first;
+                it = perStep.emplace(x.step, std::make_pair(x.bucket, unmeasured)).first;
             it->second.second[variant_slot(x.variant)] = ms;
         }
         P.reset_events();
         // per bucket: total of each shape over the launches that admit every
         // shape the bucket's launches have in common
-        std::map<int, std::array<double, 4>> tot;
-        std::map<int, std::array<bool, 4>> common;
+        std::map<int, Times> tot;
+        std::map<int, std::array<bool, NV>> common;
         for (const auto& kv : perStep) {
             const int b = kv.second.first;
             const auto& t = kv.second.second;
             auto ci = common.find(b);
             if (ci == common.end()) {
-                common[b] = {t[0] >= 0, t[1] >= 0, t[2] >= 0, t[3] >= 0};
-                tot[b] = {0.0, 0.0, 0.0, 0.0};
+                std::array<bool, NV> c;
+                for (int i = 0; i < NV; ++i) c[i] = t[i] >= 0;
+                common[b] = c;
+                tot[b].fill(0.0);
             } else {
-                for (int i = 0; i < 4; ++i) ci->second[i] = ci->second[i] && t[i] >= 0;
+                for (int i = 0; i < NV; ++i) ci->second[i] = ci->second[i] && t[i] >= 0;
             }
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < NV; ++i)
                 if (t[i] >= 0) tot[b][i] += t[i];
         }
         std::map<int, int> before;   // a shape each bucket used before the pass
@@ -1834,11 +1850,11 @@ struct Numeric : NumericBase {
             for (const auto& kv : tot) {
                 const int b = kv.first;
                 int best = -1;
-                for (int i = 0; i < 4; ++i)
+                for (int i = 0; i < NV; ++i)
                     if (common[b][i] && (best < 0 || kv.second[i] < kv.second[best])) best = i;
                 if (best < 0) continue;
                 tt.choice[b] = GEMM_VARIANTS[best];
-                for (int i = 0; i < 4; ++i) tt.ms[b][i] = common[b][i] ? kv.second[i] : -1.0;
+                for (int i = 0; i < NV; ++i) tt.ms[b][i] = common[b][i] ? kv.second[i] : -1.0;
                 auto bi = before.find(b);
                 if (bi != before.end() && bi->second != GEMM_VARIANTS[best]) ++changed;
             }
@@ -2097,11 +2113,14 @@ void Plan::resolve_profile() {
     static const char* tracePath = std::getenv("CLIQG_PROFILE_TRACE");
     FILE* trace = tracePath ? std::fopen(tracePath, "a") : nullptr;
     for (const Pending& p : pending) {
-        float ms = 0.f;
+        float ms = 0.f, t0 = 0.f;
         CQ_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
-        if (trace)
-            std::fprintf(trace, "%s %.4f %.6e %.6e %d %d %d %d\n", p.name, ms, p.flops, p.bytes,
-                         p.ndesc, p.nwork, p.variant, p.stream);
+        if (trace) {
+            // start time from the first profiled launch of this call
+            CQ_HIP(hipEventElapsedTime(&t0, pending.front().e0, p.e0));
+            std::fprintf(trace, "%s %.4f %.6e %.6e %d %d %d %d %.4f\n", p.name, ms, p.flops,
+                         p.bytes, p.ndesc, p.nwork, p.variant, p.stream, t0);
+        }
         ProfileEntry& e = profile[p.name];
         e.launches += 1;
         e.ms += ms;
@@ -2353,9 +2372,13 @@ std::string tune_table_text() {
     char line[256];
     for (size_t b = 0; b < t.keys.size(); ++b) {
         if (t.choice[b] < 0) continue;
-        std::snprintf(line, sizeof(line), "%s %d %.4f %.4f %.4f %.4f\n", t.keys[b].c_str(),
-                      t.choice[b], t.ms[b][0], t.ms[b][1], t.ms[b][2], t.ms[b][3]);
+        std::snprintf(line, sizeof(line), "%s %d", t.keys[b].c_str(), t.choice[b]);
         out += line;
+        for (int i = 0; i < NUM_GEMM_VARIANTS; ++i) {
+            std::snprintf(line, sizeof(line), " %.4f", t.ms[b][i]);
+            out += line;
+        }
+        out += "\n";
     }
     return out;
 }

@@ -266,9 +266,10 @@ cliqg_status cliqg_plan_autotune(cliqg_plan plan, const void* d_vals, cliqg_prec
                                  cliqg_front_type front_type, int solve_rhs, int* h_buckets,
                                  int* h_changed);
 /* The tuning table as text, one line per tuned bucket: bucket, chosen shape
- * (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave) and the
- * milliseconds of the 32x32, 64x64, 128x128 and eight-wave 128x128 shapes
- * over the bucket's launches (-1 where not built). h_length gets the full
+ * (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave, 7: 32x32 with
+ * 64-deep K chunks) and the milliseconds of the 32x32, 64x64, 128x128,
+ * eight-wave 128x128 and 64-deep 32x32 shapes over the bucket's launches
+ * (-1 where not built). h_length gets the full
  * length; at most capacity - 1 characters and a terminating zero are
  * copied to h_buf (which may be NULL to query the length). */
 cliqg_status cliqg_autotune_table(char* h_buf, size_t capacity, size_t* h_length);

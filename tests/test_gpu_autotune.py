@@ -112,7 +112,9 @@ print("%d %.3e %.3e" % (info["iterations"], worst, eta))
                           ("6", "helm", 20, True, "f32", "gemm_update_w8"),
                           ("4", "lap", 24, False, "f64", "gemm_update_large"),
                           ("4", "grf", 20, False, "f32", "gemm_update_large"),
-                          ("5", "helm", 16, True, "f64", "gemm_update_small")])
+                          ("5", "helm", 16, True, "f64", "gemm_update_small"),
+                          ("7", "pml", 24, True, "f32", "gemm_update_k64"),
+                          ("7", "lap", 24, False, "f64", "gemm_update_k64")])
 def test_every_tile_shape_keeps_results(device, shape, kind, n, cplx, prec, family):
     """Each tile shape, forced on every launch that admits it
     (CLIQG_GEMM_TILE, read once per process, hence subprocesses), against

@@ -433,15 +433,14 @@ print("%.3e" % backward_error(ip, ind, v, x.cpu().numpy(), b).max())
 @pytest.mark.gpu
 def test_tuning_switches_keep_results(device):
     # the non-default schedule paths (no small tiles, no split K, no
-    # lookahead, 128-wide outer panels, 64-wide solve blocks, the
-    # register-cyclic diagonal-block kernel) solve to the same FP64 rounding
-    # level; the switches are read once per process
+    # lookahead, 128-wide outer panels, 64-wide solve blocks) solve to the
+    # same FP64 rounding level; the switches are read once per process
     import os
     import subprocess
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     env = dict(os.environ, CLIQG_SMALL_TILES="0", CLIQG_SPLIT_TILES="0", CLIQG_LOOKAHEAD="0",
-               CLIQG_OUTER_PANEL="128", CLIQG_SOLVE_BLOCK="64", CLIQG_DIAG_BLOCKED="0")
+               CLIQG_OUTER_PANEL="128", CLIQG_SOLVE_BLOCK="64")
     r = subprocess.run([sys.executable, "-c", _SWITCH_SCRIPT, root], env=env,
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]
