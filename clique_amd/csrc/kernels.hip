@@ -714,6 +714,208 @@ void launch_extend_add(const ExtendDesc<T>* d_descs, const int* d_col_start, int
 }
 
 // ---------------------------------------------------------------------------
+// Fused panel step of the LDL solves' triangular sweeps over the inverted
+// SB-wide diagonal blocks (FrontLowerForwardSolve / FrontLowerBackwardSolve,
+// lower_solve/local_front.hpp:35-82): one launch per panel of a level instead
+// of two dependent products. Every workgroup forms, for 16 right-hand sides,
+//     forward:  Y = Linv_pp X(p),         backward: Y = op(Linv_pp)^T X2(p)
+// (Linv_pp lower triangular: only the triangle's K range is read); the
+// panel's own workgroups write Y as the panel's rows of the output, the band
+// workgroups apply to 64 rows r of the band
+//     forward:  X(r) -= L(r, p) Y,        backward: X2(r) -= op(L(p, r))^T Y.
+// Y is recomputed by each workgroup (no dependency inside the launch); the
+// host keeps the band to the panels of one super-panel and updates the rows
+// past it with one grouped product per super-panel.
+// 512 threads: in the Y product wave w owns rows 16w..16w+15; in the band
+// product wave w owns rows 16(w & 3).. of the 64 and half (w >> 2) of K, the
+// two halves summed through LDS. Complex FP32 uses three real MFMAs per
+// block as the grouped GEMM does.
+// ---------------------------------------------------------------------------
+template <class T, bool BWD, bool CONJ>
+__global__ __launch_bounds__(512) void k_panel_step(const PanelStepDesc<T>* __restrict__ descs,
+                                                    const int* __restrict__ task_start, int nd) {
+    using R = typename Scalar<T>::real;
+    using A4 = typename Acc4<R>::type;
+    constexpr bool CPX = Scalar<T>::complex;
+    constexpr bool F64 = sizeof(R) == 8;
+    constexpr bool M3 = CPX && !F64 && GEMM_3M;
+    constexpr int NP = CPX ? 2 : 1;
+    constexpr int NA = M3 ? 3 : (CPX ? 2 : 1);   // accumulators per block
+    constexpr int PB = PANEL_MAX;
+    constexpr int K2 = PB / 8;                   // 4-deep steps of one K half of the band product
+    __shared__ R Bs[NP][PB][16];     // the panel's rows of the input
+    __shared__ R Ys[NP][PB][16];     // Y
+    __shared__ R Red[4][NA][64][4];  // K-half partial sums of the band product
+
+    const int t = blockIdx.x;
+    const int di = find_segment(task_start, nd, t);
+    const PanelStepDesc<T> d = descs[di];
+    const int cg = (d.k + 15) / 16;
+    const int loc = t - task_start[di];
+    const bool owner = loc < cg;
+    const int c16 = owner ? loc : (loc - cg) % cg;
+    const int rb = owner ? 0 : (loc - cg) / cg;
+    const int c0 = 16 * c16;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int sbw = d.sbw;
+    auto opc = [](T v) { return CONJ ? conj_(v) : v; };
+    // fragment layout of the 16x16x4 MFMAs: lane holds A(l % 16, k0 + l / 16),
+    // B(k0 + l / 16, l % 16) and, of the 16x16 result, rows rin(r), column l % 16
+    const int fr = lane & 15, fk = lane >> 4;
+    auto rin = [&](int r) { return F64 ? fk + 4 * r : 4 * fk + r; };
+
+    // Loads that do not depend on Y are all issued first, none behind a
+    // branch (a value merged after a branch waits for its load there; an
+    // out-of-range element reads the block's first entry and is replaced by
+    // zero when used): the panel's rows of the input, the band fragments
+    // op(L)(r, k) of this wave, and the band rows of X this wave updates.
+    T xb[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = tid + 512 * e, kr = idx >> 4, j = idx & 15;
+        const bool ok = kr < sbw && c0 + j < d.k;
+        xb[e] = gload(d.Xin + (ok ? (d.w0 + kr) + (int64_t)(c0 + j) * d.ldx : 0));
+    }
+    const int r0 = d.b0 + 64 * rb;
+    const int half = wave >> 2, blk = wave & 3;
+    const int r2 = r0 + 16 * blk + fr;
+    const int kb2 = 64 * half, ke2 = min(sbw, kb2 + 64);
+    T a2[K2];
+    unsigned ok2 = 0;
+#pragma unroll
+    for (int q = 0; q < K2; ++q) {
+        const int kA = kb2 + 4 * q + fk;
+        const bool ok = !owner && r2 < d.b1 && kA < ke2;
+        const int64_t off = BWD ? kA + (int64_t)r2 * d.ldl : r2 + (int64_t)kA * d.ldl;
+        a2[q] = gload(d.Lp + (ok ? off : 0));
+        ok2 |= ok ? 1u << q : 0u;
+    }
+    T xo[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = r0 + 16 * blk + rin(q);
+        const bool ok = !owner && half == 0 && row < d.b1 && c0 + fr < d.k;
+        xo[q] = gload(d.Xin + (ok ? row + (int64_t)(c0 + fr) * d.ldx : 0));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int idx = tid + 512 * e, kr = idx >> 4, j = idx & 15;
+        const T x = pick(kr < sbw && c0 + j < d.k, xb[e], zero<T>());
+        Bs[0][kr][j] = re(x);
+        if constexpr (CPX) Bs[NP - 1][kr][j] = im(x);
+    }
+    __syncthreads();
+
+    auto mac = [&](A4 (&acc)[3], T a, int kr, R (*B)[PB][16]) {
+        const R aR = re(a), aI = CPX ? im(a) : R(0);
+        const R bR = B[0][kr][fr], bI = CPX ? B[NP - 1][kr][fr] : R(0);
+        if constexpr (M3) {
+            acc[0] = mfma16(aR, bR, acc[0]);
+            acc[1] = mfma16(aI, bI, acc[1]);
+            acc[2] = mfma16(aR + aI, bR + bI, acc[2]);
+        } else {
+            acc[0] = mfma16(aR, bR, acc[0]);
+            if constexpr (CPX) {
+                acc[0] = mfma16(-aI, bI, acc[0]);
+                acc[1] = mfma16(aR, bI, acc[1]);
+                acc[1] = mfma16(aI, bR, acc[1]);
+            }
+        }
+    };
+    auto value = [&](const A4 (&acc)[3], int r) -> T {
+        if constexpr (M3) return make_t<T>(acc[0][r] - acc[1][r], acc[2][r] - acc[0][r] - acc[1][r]);
+        else return make_t<T>(acc[0][r], CPX ? acc[1][r] : R(0));
+    };
+
+    // Y: wave w, rows 16w..16w+15, K over the triangle in 16-deep chunks,
+    // the next chunk's fragments loaded while the current one is multiplied
+    {
+        A4 acc[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] = A4{0, 0, 0, 0};
+        const int i1 = 16 * wave + fr;
+        const int kb = BWD ? 16 * wave : 0;
+        const int ke = 16 * wave < sbw ? (BWD ? sbw : min(sbw, 16 * wave + 16)) : kb;
+        auto fetch = [&](T (&av)[4], int k0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int kA = k0 + 4 * q + fk;
+                const bool ok = i1 < sbw && kA < ke;
+                const int64_t off = BWD ? kA + (int64_t)i1 * sbw : i1 + (int64_t)kA * sbw;
+                const T v = gload(d.LW + (ok ? off : 0));
+                av[q] = pick(ok, BWD ? opc(v) : v, zero<T>());
+            }
+        };
+        T cur[4], nxt[4];
+        if (kb < ke) fetch(cur, kb);
+        for (int k0 = kb; k0 < ke; k0 += 16) {
+            if (k0 + 16 < ke) fetch(nxt, k0 + 16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) mac(acc, cur[q], k0 + 4 * q + fk, Bs);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cur[q] = nxt[q];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * wave + rin(r);
+            const T y = value(acc, r);
+            Ys[0][row][fr] = re(y);
+            if constexpr (CPX) Ys[NP - 1][row][fr] = im(y);
+            if (owner && row < sbw && c0 + fr < d.k)
+                gstore(d.Xout + (d.w0 + row) + (int64_t)(c0 + fr) * d.ldx, y);
+        }
+    }
+    if (owner) return;
+    __syncthreads();
+
+    // band rows r0 .. r0 + 63 (clipped to b1): X(r) -= op(L) Y
+    A4 acc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) acc[a] = A4{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < K2; ++q) {
+        const int k4 = kb2 + 4 * q;
+        if (k4 < ke2) {   // uniform across the wave
+            const T a = pick((ok2 >> q) & 1u, BWD ? opc(a2[q]) : a2[q], zero<T>());
+            mac(acc, a, k4 + fk, Ys);
+        }
+    }
+    if (half == 1) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Red[blk][a][lane][q] = acc[a][q];
+    }
+    __syncthreads();
+    if (half == 1) return;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[a][q] += Red[blk][a][lane][q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int row = r0 + 16 * blk + rin(q);
+        if (row >= d.b1 || c0 + fr >= d.k) continue;
+        gstore(d.Xin + row + (int64_t)(c0 + fr) * d.ldx, xo[q] - value(acc, q));
+    }
+}
+
+template <class T>
+void launch_panel_step(const PanelStepDesc<T>* d, const int* task_start, int nd, int ntasks,
+                       bool backward, bool conj, hipStream_t s) {
+    if (nd <= 0 || ntasks <= 0) return;
+    if (!backward)
+        hipLaunchKernelGGL((k_panel_step<T, false, false>), dim3(ntasks), dim3(512), 0, s, d,
+                           task_start, nd);
+    else if (conj)
+        hipLaunchKernelGGL((k_panel_step<T, true, true>), dim3(ntasks), dim3(512), 0, s, d,
+                           task_start, nd);
+    else
+        hipLaunchKernelGGL((k_panel_step<T, true, false>), dim3(ntasks), dim3(512), 0, s, d,
+                           task_start, nd);
+}
+
+// ---------------------------------------------------------------------------
 // Diagonal block: unpivoted LDL^T of a b x b block (b <= NB) in FP64 (field
 // type) and the inverse of its unit lower factor. One workgroup of 256
 // threads per front. Thread (ti, tj) = (tid % 16, tid / 16) keeps the
@@ -1315,7 +1517,9 @@ void launch_fill_int(int* p, int n, int value, hipStream_t s) {
     template void launch_gather<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,    \
                                    int, const int*, hipStream_t);                               \
     template void launch_solve_extend<T>(const SolveExtend*, const int*, int, int, T*, T*, int, \
-                                         const int*, int, hipStream_t);
+                                         const int*, int, hipStream_t);                         \
+    template void launch_panel_step<T>(const PanelStepDesc<T>*, const int*, int, int, bool, bool, \
+                                       hipStream_t);
 
 CQ_INST_T(float)
 CQ_INST_T(double)

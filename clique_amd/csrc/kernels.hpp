@@ -133,6 +133,23 @@ struct SolveExtend {
     int i0;          // first child row of this launch (its count is in row_start)
 };
 
+// One front's panel of a fused solve step (launch_panel_step): the panel's
+// rows w0 .. w0+sbw-1 and the band rows [b0, b1) of the front.
+constexpr int PANEL_MAX = 128;   // widest panel of a fused step
+template <class T>
+struct PanelStepDesc {
+    const T* LW;     // the panel's inverted unit-lower diagonal block, sbw x sbw (ld sbw)
+    const T* Lp;     // forward: L(0, w0) (element (r, k) at r + k*ldl); backward: L(w0, 0)
+                     // (element (k, r) at k + r*ldl)
+    T* Xin;          // forward: X, backward: X2 (the front's rows, ld ldx)
+    T* Xout;         // forward: X2, backward: X
+    int64_t ldx;
+    int ldl;
+    int w0, sbw;
+    int b0, b1;      // band rows
+    int k;           // right-hand sides
+};
+
 // A contiguous range to clear.
 template <class T>
 struct ZeroSeg {
@@ -301,6 +318,11 @@ void launch_gather(const SolveSeg* segs, const int* row_start, int nseg, int nro
 template <class T>
 void launch_solve_extend(const SolveExtend* d, const int* row_start, int nd, int nrows,
                          T* wpool, T* xn, int N, const int* relpool, int k, hipStream_t s);
+// Fused panel step of the LDL solves' triangular sweeps (one task per 16
+// right-hand sides of a panel, and per 64 band rows and 16 right-hand sides)
+template <class T>
+void launch_panel_step(const PanelStepDesc<T>* d, const int* task_start, int nd, int ntasks,
+                       bool backward, bool conj, hipStream_t s);
 template <class T, class V>
 void launch_diag_solve(T* xn, const V* D, int N, int k, int* d_status, hipStream_t s);
 

@@ -146,7 +146,7 @@ struct IntervalAlloc {
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
                 S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER, S_BK, S_TRINV, S_GCOLS, S_QSCALE, S_PERM,
-                S_QROWS, S_PSYM };
+                S_QROWS, S_PSYM, S_PSTEP };
 
 struct Step {
     int kind;
@@ -309,6 +309,7 @@ const char* step_name(int kind) {
         case 17: return "solve_permute";
         case 18: return "quasi_rows";
         case 19: return "permute_sym";
+        case 20: return "solve_panel";
         default: return "other";
     }
 }
@@ -521,6 +522,20 @@ struct StepBuilder {
 
 // ---------------------------------------------------------------------------
 // Numeric state for one working type T (float, double, cfloat, cdouble)
+// ---------------------------------------------------------------------------
+// Panels per super-panel of the LDL solves' fused panel steps
+// (CLIQG_SOLVE_CHAIN; 0 keeps two grouped products per panel).
+constexpr int CHAIN_DEFAULT = 8;
+constexpr long CHAIN_MAX_TASKS = 512;
+static int solve_chain() {
+    static const int v = [] {
+        const char* e = std::getenv("CLIQG_SOLVE_CHAIN");
+        const int t = e ? std::atoi(e) : CHAIN_DEFAULT;
+        return (t >= 0 && t <= 64) ? t : CHAIN_DEFAULT;
+    }();
+    return v;
+}
+
 // ---------------------------------------------------------------------------
 // Outer panel width of the front factorization (a multiple of NB).
 constexpr int OB_DEFAULT = 256;
@@ -1301,11 +1316,60 @@ struct Numeric : NumericBase {
             S.wsUsed = std::max(S.wsUsed, split_gemm(&sbl, g, ws));
         };
         const int cA = P.conjugate ? G_CONJA : 0;
+        const int chainQ = (SB <= PANEL_MAX) ? solve_chain() * SB : 0;
         for (int pass = 0; pass < 2; ++pass) {
             const bool bwd = pass == 1;
             std::vector<Step>& steps = bwd ? S.bwd : S.fwd;
             StepBuilder<T> sb{S.arena, steps};
             sb.solve = true;
+            // Fused panel steps pay where a level's chain of panels is
+            // latency-bound: a few wide fronts, at most about one workgroup
+            // per CU in the first step (at most 512 workgroups; every workgroup forms the panel's
+            // product again; on wider levels that work outweighs the saved
+            // launch)
+            auto chain_level = [&](const std::vector<int>& lv, int maxn) {
+                if (chainQ <= 0 || maxn <= SB) return false;
+                const int cg = (k + 15) / 16;
+                long tasks = 0;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    if (f.n == 0) continue;
+                    const int sbw = std::min(SB, f.n);
+                    tasks += (long)cg * (1 + (std::min(chainQ, f.n) - sbw + 63) / 64);
+                }
+                return tasks <= CHAIN_MAX_TASKS;
+            };
+            // one fused panel step of the fronts of level lv wider than w0
+            // (launch_panel_step): the panel's rows and the band rows of
+            // super-panel q0
+            auto panel_step = [&](const std::vector<int>& lv, int w0, int q0, bool back) {
+                std::vector<PanelStepDesc<T>> ds;
+                std::vector<int> st;
+                int tasks = 0;
+                double fl = 0.0;
+                const int cg = (k + 15) / 16;
+                const double cm = Scalar<T>::complex ? 4.0 : 1.0;
+                for (int s : lv) {
+                    const FrontLayout& f = P.fronts[s];
+                    if (f.n <= w0) continue;
+                    const int sbw = std::min(SB, f.n - w0);
+                    const int b0 = back ? q0 : w0 + sbw;
+                    const int b1 = back ? w0 : std::min(q0 + chainQ, f.n);
+                    const int band = b1 > b0 ? b1 - b0 : 0;
+                    ds.push_back(PanelStepDesc<T>{LW(s, w0),
+                                                  back ? L(s) + w0 : L(s) + (int64_t)w0 * f.m,
+                                                  back ? X2(s) : Xs(s), back ? Xs(s) : X2(s),
+                                                  N, f.m, w0, sbw, band ? b0 : 0, band ? b1 : 0,
+                                                  k});
+                    st.push_back(tasks);
+                    tasks += cg * (1 + (band + 63) / 64);
+                    fl += cm * 2.0 * k * ((double)sbw * (sbw + 1) / 2 + (double)band * sbw);
+                }
+                if (ds.empty()) return;
+                Step x{S_PSTEP, S.arena.add(ds), S.arena.add(st), (int)ds.size(), tasks, back,
+                       fl};
+                steps.push_back(x);
+            };
             for (int hh = 0; hh < P.numLevels; ++hh) {
                 const int h = bwd ? P.numLevels - 1 - hh : hh;
                 const auto& lv = P.levels[h];
@@ -1397,9 +1461,29 @@ struct Numeric : NumericBase {
                         sb.copy(cs);
                         split(sb, g);
                     }
+                    // Fused panel steps for levels with fronts wider than one
+                    // panel: per super-panel of Q columns, one launch per panel
+                    // (X2_p := inv(L_pp) X_p, band rows of the super-panel
+                    // X(r) -= L(r, p) X2_p), then one product for the rows past
+                    // the super-panel with K = Q
+                    const bool fused = !blk && chain_level(lv, maxn);
+                    for (int q0 = 0; fused && q0 < maxn; q0 += chainQ) {
+                        for (int w0 = q0; w0 < std::min(q0 + chainQ, maxn); w0 += SB)
+                            panel_step(lv, w0, q0, false);
+                        std::vector<GemmDesc<T>> bulk;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.n <= q0 + chainQ) continue;
+                            bulk.push_back(GemmDesc<T>{L(s) + (q0 + chainQ) + (int64_t)q0 * f.m,
+                                                       X2(s) + q0, Xs(s) + q0 + chainQ, nullptr,
+                                                       nullptr, f.m, N, N, f.n - q0 - chainQ, k,
+                                                       chainQ, G_ACCUM | G_NEG});
+                        }
+                        split(sb, bulk);
+                    }
                     // SB-wide panels: X2_p := inv(L_pp) X_p (into the second
                     // buffer), X(below) -= L(below, p) X2_p
-                    for (int w0 = 0; w0 < (blk ? 0 : maxn); w0 += SB) {
+                    for (int w0 = 0; w0 < ((blk || fused) ? 0 : maxn); w0 += SB) {
                         std::vector<GemmDesc<T>> tr, up;
                         for (int s : lv) {
                             const FrontLayout& f = P.fronts[s];
@@ -1463,9 +1547,31 @@ struct Numeric : NumericBase {
                         continue;
                     }
                     split(sb, g);
+                    // Fused panel steps (as in the forward sweep), super-panels
+                    // and their panels from the last: X_p := inv(L_pp)^T X2_p,
+                    // band rows X2(r) -= L(p, r)^T X_p for r in the super-panel,
+                    // then X2(0:q0) -= L(q0:q1, 0:q0)^T X(q0:q1)
+                    const bool fused = chain_level(lv, maxn);
+                    const int maxq = fused ? (maxn + chainQ - 1) / chainQ : 0;
+                    for (int qi = maxq - 1; qi >= 0; --qi) {
+                        const int q0 = qi * chainQ, q1 = std::min(q0 + chainQ, maxn);
+                        for (int w0 = q0 + ((q1 - q0 - 1) / SB) * SB; w0 >= q0; w0 -= SB)
+                            panel_step(lv, w0, q0, true);
+                        if (q0 == 0) continue;
+                        std::vector<GemmDesc<T>> bulk;
+                        for (int s : lv) {
+                            const FrontLayout& f = P.fronts[s];
+                            if (f.n <= q0) continue;
+                            bulk.push_back(GemmDesc<T>{L(s) + q0, Xs(s) + q0, X2(s), nullptr,
+                                                       nullptr, f.m, N, N, q0, k,
+                                                       std::min(f.n, q0 + chainQ) - q0,
+                                                       G_TRANSA | G_ACCUM | G_NEG | cA});
+                        }
+                        split(sb, bulk);
+                    }
                     // SB-wide panels from the last: X_p := inv(L_pp)^T X2_p (^H if
                     // Hermitian; back into xn), X2(0:w0) -= L(p, 0:w0)^T X_p
-                    const int maxw = (maxn + SB - 1) / SB;
+                    const int maxw = fused ? 0 : (maxn + SB - 1) / SB;
                     for (int pw = maxw - 1; pw >= 0; --pw) {
                         const int w0 = pw * SB;
                         std::vector<GemmDesc<T>> tr, up;
@@ -1552,6 +1658,10 @@ struct Numeric : NumericBase {
             case S_PSYM:
                 launch_permute_sym<T>(a.at<PermSymDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
                                       s.nwork, st);
+                break;
+            case S_PSTEP:
+                launch_panel_step<T>(a.at<PanelStepDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
+                                     s.nwork, s.backward, P.conjugate, st);
                 break;
             case S_PERM:
                 // gather (variant 0) or scatter (1) of the fronts' rows from
