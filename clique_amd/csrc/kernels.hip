@@ -1,6 +1,7 @@
 // Device kernels of libcliqg for gfx950 (CDNA4). See kernels.hpp.
 #include "kernels.hpp"
 
+#include <cstdlib>
 #include <type_traits>
 
 namespace cliqg {
@@ -1047,10 +1048,214 @@ __attribute__((amdgpu_waves_per_eu(1, Scalar<T>::complex ? 4 : 8))) void k_diag_
         }
 }
 
+// ---------------------------------------------------------------------------
+// Diagonal block, sub-panel form: the same outputs as k_diag_ldl (unpivoted
+// LDL^T of a b x b block in FP64 field type, L back into the front, D, the
+// inverse of the unit lower factor) computed with 16-column sub-panels. The
+// block lives in LDS (column-major, ld NB + 1, field type). Per sub-panel p:
+//   1. one wave factors the (64 - 16p) x 16 panel with one row per lane: the
+//      pivot and the panel's column j reach every lane by cross-lane reads
+//      (v_readlane), so the 16 pivot steps need no barrier and no LDS round
+//      trip; L (scaled) goes back into the block, W = L D of the rows below
+//      the sub-panel into a side buffer;
+//   2. all four waves apply the rank-16 update A22 -= L21 op(W21)^T to the
+//      trailing lower triangle in 2 x 2 register tiles.
+// The inverse X = L^{-1} is built by blocks of 16: each diagonal block by
+// the panel wave along with its panel (the same row eliminations applied to
+// the identity), the blocks below by distance from the diagonal,
+// X_qp = -X_qq (sum_{p<=r<q} L_qr X_rp). X's
+// strict lower triangle is kept transposed in the block's free upper
+// triangle. Rows and columns >= b are padded with the identity.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double rdlane(double v, int l) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ cdouble rdlane(cdouble v, int l) {
+    return cdouble{rdlane(v.x, l), rdlane(v.y, l)};
+}
+
+// Four workgroups per CU (40 KB of LDS for real blocks; the complex
+// instantiations, 78 KB, are not launched).
+template <class T, bool HERM>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(Scalar<T>::complex ? 2 : 4, 8))) void k_diag_ldl_sp(const DiagDesc<T>* __restrict__ descs,
+                                                     int* status) {
+    using W = typename Scalar<T>::wide;
+    constexpr int LDA = NB + 1;
+    constexpr int PW = 16;            // sub-panel width
+    constexpr int WR = NB - PW;       // rows of W kept per sub-panel column (rows >= 16)
+    __shared__ W A[NB * LDA];
+    __shared__ W Wp[PW * WR];         // W = L D below the sub-panel; products of the inverse
+    __shared__ W dval[NB];
+
+    __builtin_amdgcn_s_setprio(3);
+    const DiagDesc<T> d = descs[blockIdx.x];
+    const int b = d.b, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    {
+        // lower triangle of the block, identity outside b x b; sixteen loads
+        // in flight per thread (rows along the lanes: coalesced columns)
+        T raw[NB * NB / 256];
+#pragma unroll
+        for (int t = 0; t < NB * NB / 256; ++t) {
+            const int e = tid + 256 * t, i = e & (NB - 1), c = e / NB;
+            const bool in = i < b && c < b && i >= c;
+            raw[t] = gload(d.A11 + (in ? i + (int64_t)c * d.lda : 0));
+        }
+#pragma unroll
+        for (int t = 0; t < NB * NB / 256; ++t) {
+            const int e = tid + 256 * t, i = e & (NB - 1), c = e / NB;
+            W v{};
+            if (i >= c) {
+                if (i < b && c < b) v = cvt<W>(raw[t]);
+                else if (i == c) v = one<W>();
+            }
+            A[c * LDA + i] = v;
+        }
+    }
+    __syncthreads();
+    const int np = (b + PW - 1) / PW;
+    for (int p = 0; p < np; ++p) {
+        const int c0 = PW * p;
+        if (wv == 0) {
+            const int i = c0 + lane;             // this lane's row
+            const bool act = i < NB;
+            // r: this row's panel entries; xr: for the sub-block's rows
+            // (lanes < 16), the row of X_pp = L_pp^{-1}, built by the same
+            // eliminations (row_i -= L(i, j) row_j for i > j)
+            W r[PW], xr[PW];
+#pragma unroll
+            for (int q = 0; q < PW; ++q) {
+                r[q] = act ? A[(c0 + q) * LDA + (act ? i : 0)] : W{};
+                xr[q] = (lane == q) ? one<W>() : W{};
+            }
+#pragma unroll
+            for (int j = 0; j < PW; ++j) {
+                W piv = rdlane(r[j], j);
+                if (HERM) piv = real_part_only(piv);
+                const W ip = pivot_inverse(piv);
+                const W x = r[j] * ip;           // L(i, c0 + j) for i > c0 + j
+#pragma unroll
+                for (int c = j + 1; c < PW; ++c) {
+                    const W a = rdlane(r[j], c);  // unscaled A(c0 + c, c0 + j)
+                    r[c] -= x * (HERM ? conj_(a) : a);
+                }
+                const W xl = (lane > j) ? x : W{};
+#pragma unroll
+                for (int c = 0; c <= j; ++c) xr[c] -= xl * rdlane(xr[c], j);
+                if (act && i > c0 + j) {
+                    A[(c0 + j) * LDA + i] = x;
+                    if (i >= PW) Wp[j * WR + (i - PW)] = r[j];
+                }
+                if (lane == 0) dval[c0 + j] = piv;
+            }
+            // X_pp's strict lower triangle, transposed into the block's upper one
+#pragma unroll
+            for (int c = 0; c < PW; ++c)
+                if (c < lane && lane < PW) A[(c0 + lane) * LDA + c0 + c] = xr[c];
+        }
+        __syncthreads();
+        // trailing update of rows and columns s0..63 (lower triangle) in 2 x 2 tiles
+        const int s0 = c0 + PW, h = (NB - s0) / 2;
+        for (int t = tid; t < h * (h + 1) / 2; t += 256) {
+            int ti = (int)((__fsqrt_rn(8.f * t + 1.f) - 1.f) * 0.5f);
+            while (ti * (ti + 1) / 2 > t) --ti;
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            const int tc = t - ti * (ti + 1) / 2;
+            const int i0 = s0 + 2 * ti, k0 = s0 + 2 * tc;
+            W a00 = A[k0 * LDA + i0], a10 = A[k0 * LDA + i0 + 1];
+            W a01 = A[(k0 + 1) * LDA + i0], a11 = A[(k0 + 1) * LDA + i0 + 1];
+#pragma unroll 4
+            for (int j = 0; j < PW; ++j) {
+                const W l0 = A[(c0 + j) * LDA + i0], l1 = A[(c0 + j) * LDA + i0 + 1];
+                W w0 = Wp[j * WR + (k0 - PW)], w1 = Wp[j * WR + (k0 + 1 - PW)];
+                if (HERM) {
+                    w0 = conj_(w0);
+                    w1 = conj_(w1);
+                }
+                a00 -= l0 * w0;
+                a10 -= l1 * w0;
+                a01 -= l0 * w1;
+                a11 -= l1 * w1;
+            }
+            A[k0 * LDA + i0] = a00;
+            A[k0 * LDA + i0 + 1] = a10;
+            if (ti != tc) A[(k0 + 1) * LDA + i0] = a01;
+            A[(k0 + 1) * LDA + i0 + 1] = a11;
+        }
+        __syncthreads();
+    }
+    // (the diagonal blocks of X = L^{-1} were built by the panel wave)
+    // blocks below the diagonal by distance: T = sum_r L_qr X_rp, X_qp = -X_qq T
+    for (int dist = 1; dist < np; ++dist) {
+        const int ei = tid & (PW - 1), ec = tid >> 4;
+        for (int p = 0; p + dist < np; ++p) {
+            const int q = p + dist;
+            W s{};
+            for (int rr = p; rr < q; ++rr) {
+#pragma unroll
+                for (int k = 0; k < PW; ++k) {
+                    const int row = PW * rr + k, col = PW * p + ec;
+                    // X(row, col): unit diagonal, zero above, transposed storage below
+                    W xv = (row > col) ? A[row * LDA + col] : ((row == col) ? one<W>() : W{});
+                    s += A[row * LDA + PW * q + ei] * xv;   // L(16q + ei, row)
+                }
+            }
+            Wp[p * PW * PW + ec * PW + ei] = s;
+        }
+        __syncthreads();
+        for (int p = 0; p + dist < np; ++p) {
+            const int q = p + dist;
+            W s = Wp[p * PW * PW + ec * PW + ei];
+#pragma unroll
+            for (int k = 0; k < PW; ++k)
+                if (k < ei) s += A[(PW * q + ei) * LDA + PW * q + k] * Wp[p * PW * PW + ec * PW + k];
+            A[(PW * q + ei) * LDA + PW * p + ec] = -s;
+        }
+        __syncthreads();
+    }
+    W* Dout = reinterpret_cast<W*>(d.D);
+    for (int j = tid; j < b; j += 256) {
+        Dout[j] = dval[j];
+        if (is_zero_or_nonfinite(dval[j])) record_status(status, 2, d.front, d.col0 + j);
+    }
+#pragma unroll 1
+    for (int t = 0; t < NB * NB / 256; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), c = e / NB;
+        if (i < b && c < b && i >= c)
+            d.A11[i + (int64_t)c * d.lda] = (i == c) ? one<T>() : cvt<T>(A[c * LDA + i]);
+        T li = zero<T>();
+        if (i == c) li = (i < b) ? one<T>() : zero<T>();
+        else if (i > c && i < b) li = cvt<T>(A[i * LDA + c]);
+        d.Linv[i + c * NB] = li;
+    }
+}
+
 template <class T>
 void launch_diag_ldl(const DiagDesc<T>* d_descs, int ndesc, bool herm, int* d_status,
                      hipStream_t s) {
     if (ndesc <= 0) return;
+    // Real blocks take the sub-panel kernel (one 64-block: 38 us against 47;
+    // CLIQG_DIAG_KERNEL=0 selects the register-cyclic one). Complex blocks
+    // keep the register-cyclic kernel: the sub-panel one needs 78 KB of LDS
+    // and then waits, on CUs full of the lookahead's GEMM tiles, for a CU to
+    // free that much (120^3 PML factor: 126 ms of diagonal blocks against 48).
+    if constexpr (!Scalar<T>::complex) {
+        static const bool sp = [] {
+            const char* e = std::getenv("CLIQG_DIAG_KERNEL");
+            return !(e && std::atoi(e) == 0);
+        }();
+        if (sp) {
+            if (herm)
+                hipLaunchKernelGGL((k_diag_ldl_sp<T, true>), dim3(ndesc), dim3(256), 0, s,
+                                   d_descs, d_status);
+            else
+                hipLaunchKernelGGL((k_diag_ldl_sp<T, false>), dim3(ndesc), dim3(256), 0, s,
+                                   d_descs, d_status);
+            return;
+        }
+    }
     if (herm)
         hipLaunchKernelGGL((k_diag_ldl<T, true>), dim3(ndesc), dim3(256), 0, s, d_descs, d_status);
     else
