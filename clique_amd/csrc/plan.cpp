@@ -2393,20 +2393,48 @@ void Plan::refine(const void* d_b, int64_t ldb, void* d_x, int64_t ldx, int k, d
                                   hipMemcpyDeviceToHost, stream));
             CQ_HIP(hipStreamSynchronize(stream));
             eta = 0.0;
+            std::vector<int> act;   // columns still above the tolerance
             for (int j = 0; j < k; ++j) {
                 double bn, xn, rn;
                 std::memcpy(&bn, &h[1 + j], 8);
                 std::memcpy(&xn, &h[1 + k + j], 8);
                 std::memcpy(&rn, &h[1 + 2 * k + j], 8);
                 const double den = normA * xn + bn;
-                const double e = den > 0 ? rn / den : 0.0;
-                eta = std::max(eta, std::isfinite(e) ? e : INFINITY);
+                double e = den > 0 ? rn / den : 0.0;
+                if (!std::isfinite(e)) e = INFINITY;
+                eta = std::max(eta, e);
+                if (!(e <= tol)) act.push_back(j);
             }
             if (eta <= tol) { conv = true; break; }
             if (it >= maxIters) break;
-            nm->solve_core(S, k);
-            CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
-            launch_push_add_abs<T, V>(xnb, dMap.as<int>(), x, ldx, N, k, nrm + 1 + k, stream);
+            // Converged columns keep their x: once at least 16 columns have
+            // converged, the correction solve takes only the others, gathered
+            // into a narrower solve (width rounded up to a multiple of 16, so
+            // few distinct schedules are built; padding columns are zero).
+            const int kc = (int)act.size();
+            const int kr = std::min(k, (kc + 15) / 16 * 16);
+            if (kr < k) {
+                auto& S2 = nm->schedule(kr);
+                for (int c = 0; c < kc; ++c)
+                    CQ_HIP(hipMemcpyAsync(S2.x1 + (int64_t)c * N, xnb + (int64_t)act[c] * N,
+                                          sizeof(T) * N, hipMemcpyDeviceToDevice, stream));
+                if (kr > kc)
+                    CQ_HIP(hipMemsetAsync(S2.x1 + (int64_t)kc * N, 0, sizeof(T) * N * (kr - kc),
+                                          stream));
+                nm->solve_core(S2, kr);
+                CQ_HIP(hipMemsetAsync(nrm + 1 + 2 * k, 0, sizeof(unsigned long long) * k, stream));
+                for (int c = 0; c < kc; ++c) {
+                    CQ_HIP(hipMemsetAsync(nrm + 1 + k + act[c], 0, sizeof(unsigned long long),
+                                          stream));
+                    launch_push_add_abs<T, V>(S2.x1 + (int64_t)c * N, dMap.as<int>(),
+                                              x + (int64_t)act[c] * ldx, ldx, N, 1,
+                                              nrm + 1 + k + act[c], stream);
+                }
+            } else {
+                nm->solve_core(S, k);
+                CQ_HIP(hipMemsetAsync(nrm + 1 + k, 0, sizeof(unsigned long long) * 2 * k, stream));
+                launch_push_add_abs<T, V>(xnb, dMap.as<int>(), x, ldx, N, k, nrm + 1 + k, stream);
+            }
             ++it;
         }
         *itersOut = it;

@@ -180,6 +180,32 @@ def test_fp32_factor_with_fp64_refinement(device, kind, n, cut, cplx):
 
 
 @pytest.mark.gpu
+def test_refinement_skips_converged_columns(device):
+    """Columns that reach the tolerance drop out of the correction solves:
+    with 24 zero right-hand sides (converged at the first residual) among 40,
+    every correction solve runs on the 16 others, gathered into a narrower
+    solve. Each column must still reach the tolerance, the zero columns stay
+    zero, and the others agree with a refinement of those 16 alone."""
+    ip, ind, v, dims = problem("pml", 16)
+    N = ip.size - 1
+    sym, ft = gpu_tree(ip, ind, v, dims, 64, is_complex=True, precision="f32")
+    rng = np.random.default_rng(11)
+    live = np.sort(rng.choice(40, 16, replace=False))
+    b = np.zeros((N, 40), dtype=np.complex128)
+    b[:, live] = rng.standard_normal((N, 16)) + 1j * rng.standard_normal((N, 16))
+    x, info = ft.refine(to_dev(b), tol=1e-10, max_iters=REFINE_CAP)
+    xg = x.cpu().numpy()
+    eta = backward_error(ip, ind, v, xg[:, live], b[:, live])
+    assert info["converged"] and eta.max() <= 1e-10, (info, eta)
+    assert info["iterations"] >= 1
+    dead = np.setdiff1d(np.arange(40), live)
+    assert np.all(xg[:, dead] == 0)
+    x16, info16 = ft.refine(to_dev(b[:, live]), tol=1e-10, max_iters=REFINE_CAP)
+    assert info16["iterations"] == info["iterations"]
+    assert rel_diff(xg[:, live], x16.cpu().numpy()) <= 1e-8
+
+
+@pytest.mark.gpu
 def test_multiply_matches_oracle(device):
     ip, ind, v, dims = problem("helm", 10)
     N = ip.size - 1
