@@ -436,7 +436,8 @@ def main():
     # where the flop convention counts four real multiplications
     mfma_share = 0.75 if cplx else 1.0
     traffic, traffic_src = measured_traffic(args.config, args.n, upd_name)
-    solve_names = ("solve_gemm", "solve_extend_add", "solve_gather", "solve_zero", "solve_reduce")
+    solve_names = ("solve_gemm", "solve_gemm_stream", "solve_extend_add", "solve_gather",
+                   "solve_zero", "solve_reduce", "solve_panel")
 
     def families(prof, solve):
         # one step (factor + refinement)

@@ -349,8 +349,8 @@ class DistSymmFrontTree:
         return v.value
 
 
-_SHAPES = {0: "64x64", 4: "128x128", 5: "32x32", 6: "128x128x8w", 7: "32x32k64"}
-_SHAPE_ORDER = ("32x32", "64x64", "128x128", "128x128x8w", "32x32k64")   # GEMM_VARIANTS
+_SHAPES = {0: "64x64", 4: "128x128", 5: "32x32", 6: "128x128x8w", 7: "32x32k64", 8: "stream"}
+_SHAPE_ORDER = ("32x32", "64x64", "128x128", "128x128x8w", "32x32k64", "stream")   # GEMM_VARIANTS
 
 
 def autotune_table():

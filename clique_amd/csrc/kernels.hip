@@ -166,11 +166,13 @@ __device__ __forceinline__ void group_tile(int t, int tm, int tn, bool lower, in
 }  // namespace
 
 int gemm_tile_m(int variant) {
+    if (variant == TILE_STREAM) return 64 * STREAM_ROW_BLOCKS;
     return (variant == TILE_32 || variant == TILE_32_K64)
                ? 32
                : ((variant == TILE_128 || variant == TILE_128_8W) ? 128 : 64);
 }
-int gemm_tile_n(int variant) { return gemm_tile_m(variant); }
+// a streaming tile takes all N <= 64 columns
+int gemm_tile_n(int variant) { return variant == TILE_STREAM ? 64 : gemm_tile_m(variant); }
 
 int gemm_tiles(int M, int N, int flags, int variant) {
     if (M <= 0 || N <= 0) return 0;
@@ -496,6 +498,217 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 2048) ? 2
             }
 }
 
+// eight consecutive FP32-type entries through 16-byte global loads
+__device__ __forceinline__ void gload8(const float* p, float* dst) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const __attribute__((address_space(1))) f4* q = (const __attribute__((address_space(1))) f4*)(p);
+    const f4 a = q[0], b = q[1];
+    dst[0] = a.x; dst[1] = a.y; dst[2] = a.z; dst[3] = a.w;
+    dst[4] = b.x; dst[5] = b.y; dst[6] = b.z; dst[7] = b.w;
+}
+__device__ __forceinline__ void gload8(const cfloat* p, cfloat* dst) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const __attribute__((address_space(1))) f4* q = (const __attribute__((address_space(1))) f4*)(p);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const f4 a = q[i];
+        dst[2 * i] = cfloat{a.x, a.y};
+        dst[2 * i + 1] = cfloat{a.z, a.w};
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming product of the solves (tile shape TILE_STREAM, FP32 working
+// types): C (M x N, N <= 64) (+)= +-op(A) B, where A is the factor (read once
+// per solve) and B a block of right-hand sides. A workgroup owns 64 rows of C
+// and all of its columns; wave w the rows 16w..16w+15. A goes from global
+// memory straight into the MFMA operand registers (no LDS staging, no
+// barrier on its path); only the K x N block of B passes through LDS, where
+// the four waves share it. K runs in chunks of 32, permuted inside a chunk
+// so that a lane's eight elements of op(A) are consecutive in k: MFMA block
+// s (k-step of 4) takes k = k0 + 8 g + s from lane group g = lane / 16, for
+// A and B alike, which leaves the sum unchanged. For op(A) = A^T (the
+// backward sweep) those eight elements are contiguous in memory. The next
+// chunk of A and B is loaded while the current one feeds the MFMAs.
+// Complex products use three real MFMAs per block (Gauss), B kept in LDS as
+// planes re, im and re + im.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_gemm_stream(
+    const GemmDesc<T>* __restrict__ descs, const int* __restrict__ tile_start, int ndesc) {
+    static_assert(sizeof(typename Scalar<T>::real) == 4, "FP32 working types only");
+    constexpr bool CPX = Scalar<T>::complex;
+    constexpr int NPL = CPX ? 3 : 1;   // planes of B: re (, im, re + im)
+    constexpr int KC = 32;             // chunk depth
+    constexpr int KS = KC + 4;         // LDS row stride (floats) of a column of B
+    constexpr int RB = STREAM_ROW_BLOCKS;   // 16-row blocks per wave
+    __shared__ float Bs[2][NPL][64][KS];
+
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int di = find_segment(tile_start, ndesc, tile);
+    const GemmDesc<T> d = descs[di];
+    const int row0 = (tile - tile_start[di]) * (64 * RB);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int g = lane >> 4, li = lane & 15;
+    const int ncb = (d.N + 15) >> 4;   // 16-column blocks of C
+    const bool transA = d.flags & G_TRANSA, conjA = d.flags & G_CONJA;
+    const int K = d.K;
+
+    // this lane's rows of op(A): op(A)(r, k) = A[r + k lda], or A[k + r lda]
+    const int wrow0 = row0 + wave * 16 * RB;
+    bool rowOK[RB];
+    const T* pa[RB];
+#pragma unroll
+    for (int b = 0; b < RB; ++b) {
+        const int arow = wrow0 + 16 * b + li;
+        rowOK[b] = arow < d.M;
+        pa[b] = d.A + (rowOK[b] ? (transA ? (int64_t)arow * d.lda : (int64_t)arow) : 0);
+    }
+    const int64_t astep = transA ? 1 : d.lda;
+    // B: thread t loads k = 8 (t & 3) .. +7 of column t >> 2
+    const int bj = tid >> 2, bq = tid & 3;
+    const bool colOK = bj < d.N;
+    const T* pb = d.B + (colOK ? (int64_t)bj * d.ldb : 0);
+
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 acc[4][RB][NPL];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int p = 0; p < NPL; ++p) acc[c][b][p] = f4{0.f, 0.f, 0.f, 0.f};
+
+    T ra[RB][8], rn[RB][8], rb[8];
+    // Loads never branch per element: an element outside the matrix reads
+    // the operand's first entry and becomes zero afterwards.
+    auto loadA = [&](int k0, T (*dst)[8]) {
+        const int kb = k0 + 8 * g;
+#pragma unroll
+        for (int b = 0; b < RB; ++b) {
+            if (rowOK[b] && kb + 8 <= K) {
+                if (transA) {
+                    gload8(pa[b] + kb, dst[b]);   // eight consecutive entries of a column of A
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) dst[b][s] = gload(pa[b] + (int64_t)(kb + s) * d.lda);
+                }
+                continue;
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const bool ok = rowOK[b] && kb + s < K;
+                dst[b][s] = gload(pa[b] + (ok ? (int64_t)(kb + s) * astep : 0));
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                if (!(rowOK[b] && kb + s < K)) dst[b][s] = zero<T>();
+        }
+    };
+    auto loadB = [&](int k0) {
+        const int kb = k0 + 8 * bq;
+        if (colOK && kb + 8 <= K) {
+            gload8(pb + kb, rb);
+            return;
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const bool ok = colOK && kb + s < K;
+            rb[s] = gload(pb + (ok ? kb + s : 0));
+        }
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (!(colOK && kb + s < K)) rb[s] = zero<T>();
+    };
+    auto storeB = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            Bs[buf][0][bj][8 * bq + s] = re(rb[s]);
+            if constexpr (CPX) {
+                Bs[buf][1][bj][8 * bq + s] = im(rb[s]);
+                Bs[buf][2][bj][8 * bq + s] = re(rb[s]) + im(rb[s]);
+            }
+        }
+    };
+
+    loadA(0, ra);
+    loadB(0);
+    storeB(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        const bool more = k0 + KC < K;
+        if (more) {
+            loadA(k0 + KC, rn);
+            loadB(k0 + KC);
+        }
+        // s outer, column blocks inside: 4 x RB x 3 independent accumulators
+        // between dependent MFMAs, few B values live at a time
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            float aR[RB], aI[RB];
+#pragma unroll
+            for (int b = 0; b < RB; ++b) {
+                aR[b] = re(ra[b][s]);
+                if constexpr (CPX) aI[b] = conjA ? -im(ra[b][s]) : im(ra[b][s]);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (c >= ncb) break;
+                const float bR = Bs[buf][0][16 * c + li][8 * g + s];
+                if constexpr (CPX) {
+                    const float bI = Bs[buf][1][16 * c + li][8 * g + s];
+                    const float bS = Bs[buf][2][16 * c + li][8 * g + s];
+#pragma unroll
+                    for (int b = 0; b < RB; ++b) {
+                        acc[c][b][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[b], bR, acc[c][b][0], 0, 0, 0);
+                        acc[c][b][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(aI[b], bI, acc[c][b][1], 0, 0, 0);
+                        acc[c][b][2] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[b] + aI[b], bS, acc[c][b][2], 0, 0, 0);
+                    }
+                } else {
+#pragma unroll
+                    for (int b = 0; b < RB; ++b)
+                        acc[c][b][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[b], bR, acc[c][b][0], 0, 0, 0);
+                }
+            }
+        }
+        if (more) {
+            storeB(buf ^ 1);
+#pragma unroll
+            for (int b = 0; b < RB; ++b)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) ra[b][s] = rn[b][s];
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+    // element r of lane l of a 16x16 block is C(4 (l / 16) + r, l % 16)
+    const bool neg = d.flags & G_NEG, accum = d.flags & G_ACCUM;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (c >= ncb) break;
+        const int col = 16 * c + li;
+#pragma unroll
+        for (int b = 0; b < RB; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = wrow0 + 16 * b + 4 * g + r;
+                if (row >= d.M || col >= d.N) continue;
+                T v;
+                if constexpr (CPX) {
+                    const float p1 = acc[c][b][0][r], p2 = acc[c][b][1][r], p3 = acc[c][b][2][r];
+                    v = make_t<T>(p1 - p2, p3 - p1 - p2);
+                } else {
+                    v = make_t<T>(acc[c][b][0][r], 0.f);
+                }
+                if (neg) v = -v;
+                T* pc = d.C + row + (int64_t)col * d.ldc;
+                if (accum) v = *pc + v;
+                *pc = v;
+            }
+    }
+}
+
 template <class T, int TM, int TN, int KT = 16, int WM = 2, int WN = 2>
 static void launch_gemm_shape(const GemmDesc<T>* d, const int* ts, int nd, int nt, int role,
                               hipStream_t s) {
@@ -524,6 +737,12 @@ void launch_gemm(const GemmDesc<T>* d_descs, const int* d_tile_start, int ndesc,
             launch_gemm_shape<T, 128, 128, 16, 2, 4>(d_descs, d_tile_start, ndesc, ntiles, role, s);
         else
             throw Error(1, "eight-wave 128x128 GEMM tiles are built for complex FP32 only");
+    } else if (variant == TILE_STREAM) {
+        if constexpr (gemm_variant_allowed<T>(TILE_STREAM))
+            hipLaunchKernelGGL((k_gemm_stream<T>), dim3(ntiles), dim3(256), 0, s, d_descs,
+                               d_tile_start, ndesc);
+        else
+            throw Error(1, "streaming GEMM tiles are built for FP32 working types only");
     } else if (variant == TILE_32) {
         launch_gemm_shape<T, 32, 32>(d_descs, d_tile_start, ndesc, ntiles, role, s);
     } else if (variant == TILE_32_K64) {

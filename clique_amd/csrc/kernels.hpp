@@ -19,6 +19,8 @@ enum GemmTile : int {
     TILE_128_8W = 6,  // 128x128 tiles, eight waves of 64x32 (complex FP32)
     TILE_32_K64 = 7,  // 32x32 tiles with 64-deep K chunks (latency-bound launches
                       // with small K: one global round trip per 64 of K, not per 16)
+    TILE_STREAM = 8,  // solve products with N <= 64 (FP32 types): 64 rows by all
+                      // columns, op(A) streamed from global memory into the MFMAs
 };
 int gemm_tile_m(int variant);
 int gemm_tile_n(int variant);
@@ -33,12 +35,13 @@ constexpr bool gemm_variant_allowed(int variant) {
     if (variant == TILE_128_8W) return Scalar<T>::complex && sizeof(T) == 8;
     // 64-deep chunks of complex FP64 would need 192 KB of LDS
     if (variant == TILE_32_K64) return !(Scalar<T>::complex && sizeof(T) == 16);
+    if (variant == TILE_STREAM) return sizeof(typename Scalar<T>::real) == 4;
     return variant == TILE_64 || variant == TILE_32;
 }
 // Every tile shape, in the order the autotuner tries them
-constexpr int NUM_GEMM_VARIANTS = 5;
+constexpr int NUM_GEMM_VARIANTS = 6;
 constexpr int GEMM_VARIANTS[NUM_GEMM_VARIANTS] = {TILE_32, TILE_64, TILE_128, TILE_128_8W,
-                                                  TILE_32_K64};
+                                                  TILE_32_K64, TILE_STREAM};
 
 enum GemmFlags : int {
     G_TRANSA = 1,   // op(A)(i,k) = A[k + i*lda]   (else A[i + k*lda])
@@ -53,6 +56,10 @@ enum GemmFlags : int {
     G_KHI = 512,    // op(A)(i,k) = 0 for k > i: a row tile stops K after its last row
     G_SYMM = 1024,  // with G_LOWER: also write C(j,i) = C(i,j) (conjugated with G_CONJA)
 };
+// 16-row blocks per wave of a streaming tile (four waves: 64 x this rows)
+constexpr int STREAM_ROW_BLOCKS = 1;   // two measured slower on the 120^3 PML solves (28.9 ms vs 24.7)
+// Flags a streaming tile accepts (a plain or transposed A, no masks)
+constexpr int STREAM_FLAGS = G_TRANSA | G_CONJA | G_ACCUM | G_NEG;
 
 // C(MxN) (+)= +-op(A)(MxK) diag(scaleK) op(B)(KxN) [diag(scaleN)^{-1}], where
 // op(A) is A, A^T, conj(A) or A^H per the flags (scaleK is never conjugated).

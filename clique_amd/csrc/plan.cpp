@@ -226,11 +226,16 @@ void tune_load_locked(TuneTable& t) {
     }
     std::fclose(f);
 }
-int tune_bucket(int type, int role, bool lower, double tiles64, double meanK) {
+int tune_bucket(int type, int role, bool lower, double tiles64, double meanK, int n) {
     auto lg = [](double x) { return x >= 1.0 ? (int)std::lround(std::log2(x)) : 0; };
     char key[128];
-    std::snprintf(key, sizeof(key), "t%d.r%d.l%d.T%d.K%d", type, role, lower ? 1 : 0, lg(tiles64),
-                  lg(meanK));
+    // solve products also by their right-hand-side count (16-column blocks)
+    if (role == 1)
+        std::snprintf(key, sizeof(key), "t%d.r%d.l%d.T%d.K%d.N%d", type, role, lower ? 1 : 0,
+                      lg(tiles64), lg(meanK), (std::min(n, 64) + 15) / 16);
+    else
+        std::snprintf(key, sizeof(key), "t%d.r%d.l%d.T%d.K%d", type, role, lower ? 1 : 0,
+                      lg(tiles64), lg(meanK));
     TuneTable& t = tune_table();
     std::lock_guard<std::mutex> lock(t.mu);
     tune_load_locked(t);
@@ -238,7 +243,7 @@ int tune_bucket(int type, int role, bool lower, double tiles64, double meanK) {
 }
 // CLIQG_GEMM_TILE=<shape>: every launch that admits the shape takes it
 // (0: 64x64, 4: 128x128, 5: 32x32, 6: 128x128 eight-wave, 7: 32x32 with
-// 64-deep K chunks), ahead of the
+// 64-deep K chunks, 8: streaming solve tiles), ahead of the
 // tuned and the modelled choice
 int forced_variant() {
     static const int v = [] {
@@ -440,6 +445,15 @@ struct StepBuilder {
             if (v == TILE_32_K64 && !solve)
                 for (const auto& d : ds)
                     if (d.flags & (G_KLO | G_KHI)) ok = false;
+            // streaming tiles: solve products of at most 64 right-hand
+            // sides, plain or transposed A, no masks, scales or aliasing
+            if (v == TILE_STREAM) {
+                ok = ok && solve && !scaled;
+                for (const auto& d : ds)
+                    if (d.N > 64 || (d.flags & ~STREAM_FLAGS) || d.scaleN || d.C == d.A ||
+                        d.C == d.B)
+                        ok = false;
+            }
             if (!ok) continue;
             std::vector<int> start;
             int total = 0;
@@ -454,10 +468,16 @@ struct StepBuilder {
         }
         if (s.nvar == 0) throw Error(1, "an in-place GEMM needs all of its columns in one tile");
         const int role = scaled ? 2 : (upd ? 3 : (solve ? 1 : 0));
-        s.bucket = tune_bucket(type_code<T>(), role, lower, tiles64, ksum / ds.size());
+        int maxN = 0;
+        for (const auto& d : ds) maxN = std::max(maxN, d.N);
+        s.bucket = tune_bucket(type_code<T>(), role, lower, tiles64, ksum / ds.size(), maxN);
         const int forced = forced_variant();
         const int tuned = tuned_variant(s.bucket);
-        const int model = choose_variant(ds);
+        int model = choose_variant(ds);
+        // solve products of at most 32 right-hand sides stream op(A) (the
+        // 120^3 PML solve with 32: 35.7 ms with every such product
+        // streaming, 38.4 with the tile shapes)
+        if (solve && maxN <= 32 && step_has_variant(s, TILE_STREAM)) model = TILE_STREAM;
         set_variant(s, forced >= 0 && step_has_variant(s, forced)  ? forced
                        : tuned >= 0 && step_has_variant(s, tuned) ? tuned
                                                                    : model);
@@ -1738,6 +1758,7 @@ struct Numeric : NumericBase {
                                                                             : "gemm_update")
                                               : "gemm_trsm")
                                    : (s.kind == S_ZERO && S) ? "solve_zero"
+                                   : (kind == 7 && s.variant == TILE_STREAM) ? "solve_gemm_stream"
                                                              : step_name(kind);
                 P.pending.push_back(Plan::Pending{name, s.flops, s.bytes, e0, e1, s.ndesc, s.nwork,
                                                   s.variant, s.stream ? 1 : 0});

@@ -54,9 +54,11 @@ struct NumericBase {
 };
 
 // The autotuner's table as text, one line per tuned bucket: the bucket
-// (t<type>.r<role>.l<lower>.T<log2 tiles>.K<log2 mean K>), the chosen tile
+// (t<type>.r<role>.l<lower>.T<log2 tiles>.K<log2 mean K>, and for solve
+// products .N<16-column blocks of right-hand sides>), the chosen tile
 // shape and the milliseconds each shape (32x32, 64x64, 128x128 four-wave,
-// 128x128 eight-wave) took over the bucket's launches, -1 where not built.
+// 128x128 eight-wave, 32x32 with 64-deep chunks, streaming) took over the
+// bucket's launches, -1 where not built.
 std::string tune_table_text();
 
 class Plan {

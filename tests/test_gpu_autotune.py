@@ -144,3 +144,71 @@ def test_every_tile_shape_keeps_results(device, shape, kind, n, cplx, prec, fami
     else:
         assert worst <= 3 * worst0, (worst0, worst)
     assert eta <= 1e-10 and its <= its0 + 1, (its0, its, eta)
+
+
+_STREAM = r"""
+import sys, numpy as np, torch
+sys.path.insert(0, sys.argv[1])
+sys.path.insert(0, sys.argv[1] + "/tests")
+import clique_amd as cq
+import oracle
+from helpers import backward_error, hermitian_matrix, problem
+kind, n, k = sys.argv[2], int(sys.argv[3]), int(sys.argv[4])
+conj = kind == "herm"
+if conj:
+    ip, ind, v, dims = hermitian_matrix(n)
+else:
+    ip, ind, v, dims = problem(kind, n)
+cplx = np.iscomplexobj(v)
+N = ip.size - 1
+sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=64)
+ft = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx, conjugate=conj)
+ft.factor(torch.tensor(v, device="cuda:0"), precision="f32")
+rng = np.random.default_rng(7)
+b = rng.standard_normal((N, k)) + (1j * rng.standard_normal((N, k)) if cplx else 0)
+x = cq.colmajor(torch.tensor(b, device="cuda:0"))
+ft.set_profiling(True)
+ft.solve(x)
+prof = ft.profile()
+ft.set_profiling(False)
+t = oracle.Tree(*dims, ip, ind, cutoff=64)
+f = oracle.Factor(t, is_complex=cplx, conjugate=conj)
+f.initialize(ip, ind, v)
+f.ldl(nb=96)
+xo = f.solve(b)
+err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+xr, info = ft.refine(cq.colmajor(torch.tensor(b, device="cuda:0")), tol=1e-10, max_iters=8)
+eta = backward_error(ip, ind, v, xr.cpu().numpy(), b).max()
+print(sorted(prof))
+print("%d %.3e %.3e" % (info["iterations"], err, eta))
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,n,k", [("pml", 24, 64), ("pml", 24, 20), ("lap", 24, 64),
+                                      ("herm", 14, 33)])
+def test_streaming_solve_tiles_keep_results(device, kind, n, k):
+    """The streaming solve tiles (TILE_STREAM: op(A) read from global memory
+    straight into the MFMAs), forced on every solve product that admits them
+    (plain and transposed A, conjugated for the Hermitian case, N = k <= 64
+    right-hand sides), against the default shapes: one FP32 solve within
+    three times the default shapes' distance to the oracle's FP64 solution,
+    and FP64 refinement to 1e-10 in at most one step more."""
+    def run(env_extra):
+        env = dict(os.environ, **env_extra)
+        if not env_extra:
+            env.pop("CLIQG_GEMM_TILE", None)
+        r = subprocess.run([sys.executable, "-c", _STREAM, ROOT, kind, str(n), str(k)], env=env,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        lines = r.stdout.strip().splitlines()
+        its, err, eta = lines[-1].split()
+        return lines[-2], int(its), float(err), float(eta)
+
+    fams0, its0, err0, eta0 = run({})
+    fams, its, err, eta = run({"CLIQG_GEMM_TILE": "8"})
+    print(f"default: {its0} steps, solve error {err0:.2e}; streaming: {its} steps, "
+          f"solve error {err:.2e}")
+    assert "solve_gemm_stream" in fams, fams
+    assert err <= 3 * err0, (err0, err)
+    assert eta <= 1e-10 and its <= its0 + 1, (its0, its, eta)
