@@ -178,6 +178,52 @@ def measured_traffic(cfg, n, family):
     return t["bytes_per_launch"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
+TUNING_FILE = os.path.join(ROOT, "clique_amd", "tuning_mi355x.txt")
+
+
+def use_tuning(args):
+    """Point the library at a copy of the tile table (CLIQG_TUNE_CACHE, read
+    at the first plan; a copy, so that a --retune pass does not rewrite the
+    committed file). Returns the table's path relative to the repository, or
+    None (cost-model shapes, or a table set in the environment)."""
+    if args.no_tune or os.environ.get("CLIQG_TUNE_CACHE") or not os.path.exists(args.tuning):
+        return None
+    import shutil
+    import tempfile
+    fd, tmp = tempfile.mkstemp(prefix="cliqg_tune_", suffix=".txt")
+    os.close(fd)
+    shutil.copyfile(args.tuning, tmp)
+    os.environ["CLIQG_TUNE_CACHE"] = tmp
+    return os.path.relpath(os.path.abspath(args.tuning), ROOT)
+
+
+def write_tuning(path):
+    """cliqg_plan_autotune over the FP32 and FP64 plans of configurations 1-5
+    (configuration 5 last, so that the headline workload's buckets keep its
+    own choices; its FP32 plan also at 32 right-hand sides, the width of the
+    refinement's narrowed correction solves); the process-wide table is
+    written to `path` after every pass."""
+    os.environ["CLIQG_TUNE_CACHE"] = os.path.abspath(path)
+    if os.path.exists(path):
+        os.remove(path)
+    import torch
+    import clique_amd as cq
+    for cfg in (1, 2, 3, 4, 5):
+        ip, ind, v, dims, b, cplx, desc = workload(cfg)
+        sym = symbolic_for(ip, ind, dims)
+        vals = torch.tensor(v, device="cuda:0")
+        for prec in ("f32", "f64"):
+            p = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
+            widths = (b.shape[1], 32) if (cfg == 5 and prec == "f32") else (b.shape[1],)
+            for k in widths:
+                nb, nc = p.autotune(vals, precision=prec, rhs=k)
+                print(f"# config {cfg} {prec} rhs {k}: {nb} buckets, {nc} changed",
+                      file=sys.stderr, flush=True)
+            del p
+            torch.cuda.empty_cache()
+    print(json.dumps({"table": path, "buckets": len(cq.autotune_table())}))
+
+
 def factor_only(args):
     """Two profiled FP32 factorizations (launches serialized on one stream, as
     in the profiled factorization of the benchmark line) and a 1 GiB device
@@ -189,7 +235,7 @@ def factor_only(args):
     sym = cq.NaturalNestedDissection(*dims, ip, ind, cutoff=128)
     vals = torch.tensor(v, device="cuda:0")
     p = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
-    if not args.no_tune:
+    if args.retune and not args.no_tune:
         p.autotune(vals, precision="f32")
     p.set_profiling(True)
     for _ in range(2):
@@ -251,7 +297,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the GEMM tile autotuning pass (cost-model tile shapes)")
+                    help="no tuning table and no tuning pass (cost-model tile shapes)")
+    ap.add_argument("--tuning", default=TUNING_FILE, metavar="FILE",
+                    help="GEMM tile table written by cliqg_plan_autotune (bench.py "
+                         "--write-tuning) that the plans use; default: the committed MI355X table")
+    ap.add_argument("--retune", action="store_true",
+                    help="run the autotuner on both plans before the timed steps (starting "
+                         "from the table)")
+    ap.add_argument("--write-tuning", default=None, metavar="FILE",
+                    help="autotune the FP32 and FP64 plans of every configuration and write "
+                         "the table to FILE")
     ap.add_argument("--block", action="store_true",
                     help="add a BLOCK_LDL (selective inversion) FP32+IR line")
     ap.add_argument("--no-block", action="store_true", help=argparse.SUPPRESS)
@@ -267,6 +322,10 @@ def main():
         tgt = (float(ft), float(st)) if float(ft) > 0 else None
         print(json.dumps(cpu_baseline(args.cpu_budget, k=int(kk), target=tgt)))
         return
+    if args.write_tuning:
+        write_tuning(args.write_tuning)
+        return
+    tuning = use_tuning(args)
     if args.gpus != 1:
         print("bench.py measures one GPU only: the multifrontal factor and solves run on "
               "one MI355X here, and runs on several GPUs are not implemented", file=sys.stderr)
@@ -292,8 +351,8 @@ def main():
           f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
     p32 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
-    tune = {}
-    if not args.no_tune:
+    tune = {"table": tuning} if tuning else {}
+    if args.retune and not args.no_tune:
         # the project's tile autotuner, once per plan before the timed steps
         t_tune = time.perf_counter()
         nb, nc = p32.autotune(vals, precision="f32", rhs=k)
@@ -380,7 +439,7 @@ def main():
     if not args.no_fp64:
         p64 = cq.DistSymmFrontTree(sym, ip, ind, is_complex=cplx)
         x64 = cq.colmajor(torch.empty_like(bd))
-        if not args.no_tune:
+        if args.retune and not args.no_tune:
             t_tune = time.perf_counter()
             nb, nc = p64.autotune(vals, precision="f64", rhs=k)
             tune["fp64"] = {"buckets": nb, "changed": nc, "s": time.perf_counter() - t_tune}
