@@ -211,8 +211,10 @@ cliqg_status cliqg_col_norms(int N, int k, const void* d_x, int ldx, cliqg_field
 
 /* FP64 iterative refinement (no reference counterpart): x := solve(b); while
  * some column has backward error > tol and iterations < max_iters:
- * r := b - A x (FP64), x := x + solve(r) (FP64 update). b is not modified;
- * x and b must not overlap (CLIQG_ERR_INVALID otherwise). */
+ * r := b - A x (FP64), x := x + solve(r) (FP64 update). Columns already at
+ * or below tol keep their x: the correction solves take only the others
+ * (once at least 16 columns have converged). b is not modified; x and b
+ * must not overlap (CLIQG_ERR_INVALID otherwise). */
 cliqg_status cliqg_plan_refine(cliqg_plan plan, const void* d_b, int ldb, void* d_x, int ldx,
                                int k, double tol, int max_iters, cliqg_refine_info* info);
 
