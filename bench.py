@@ -190,9 +190,11 @@ def use_tuning(args):
         return None
     import shutil
     import tempfile
+    import atexit
     fd, tmp = tempfile.mkstemp(prefix="cliqg_tune_", suffix=".txt")
     os.close(fd)
     shutil.copyfile(args.tuning, tmp)
+    atexit.register(lambda: os.path.exists(tmp) and os.remove(tmp))
     os.environ["CLIQG_TUNE_CACHE"] = tmp
     return os.path.relpath(os.path.abspath(args.tuning), ROOT)
 
