@@ -879,6 +879,122 @@ void launch_assemble(const V* vals, const int64_t* dst, int64_t nnz, T* pool, bo
 }
 
 // ---------------------------------------------------------------------------
+// The solves' inverted diagonal block of a small front (n <= 2 NB) in one
+// workgroup: LW(0:n0, 0:n0) = X0, LW(0:n0, n0:n) = 0, LW(n0:n, n0:n) = X1 and
+// LW(n0:n, 0:n0) = -X1 (L10 X0), the two 64 x 64 products with both operands
+// in LDS and a 4 x 4 register tile per thread. One launch per level replaces
+// the clearing, the two block copies and the two grouped products of the
+// general path (wide blocks of larger fronts keep that path).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(256) void k_wide_inverse_small(const WideInvDesc<T>* __restrict__ descs) {
+    // rows padded to a multiple of four entries: a thread's four consecutive
+    // entries are one 16-byte aligned LDS read (or two, complex)
+    __shared__ __attribute__((aligned(16))) T Sa[NB][NB + 4];   // Sa[k][i] = A(i, k)
+    __shared__ __attribute__((aligned(16))) T Sb[NB][NB + 4];   // Sb[k][j] = B(k, j)
+    constexpr int PT = NB * NB / 256;   // entries of a 64 x 64 block per thread
+    const WideInvDesc<T> d = descs[blockIdx.x];
+    const int n = d.n, tid = threadIdx.x;
+    const int n0 = min(n, NB), b1 = n - n0;
+    // Every copy issues all of a thread's loads before its first store (a
+    // load followed by its store per iteration would wait a global round
+    // trip per entry). Element t of a thread is (i, j) = (e % 64, e / 64)
+    // with e = tid + 256 t: lanes run down a column.
+    T v[PT], w[PT];
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+        v[t] = gload(d.X0 + i + j * NB);
+    }
+    if (b1 > 0) {
+#pragma unroll
+        for (int t = 0; t < PT; ++t) {
+            const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+            w[t] = gload(d.X1 + i + j * NB);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+        if (i < n0 && j < n0) d.LW[i + (int64_t)j * n] = v[t];
+    }
+    if (b1 <= 0) return;
+    T l[PT];
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+        const bool ok = i < b1;
+        l[t] = gload(d.L10 + (ok ? i + (int64_t)j * d.ldl : 0));
+    }
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+        // zero block above X1, X1 itself
+        if (j < b1) d.LW[i + (int64_t)(NB + j) * n] = zero<T>();
+        if (i < b1 && j < b1) d.LW[(NB + i) + (int64_t)(NB + j) * n] = w[t];
+        // T1 = L10 X0: Sa[k][i] = L10(i, k) (rows >= b1 read as zero), Sb[k][j] = X0(k, j)
+        Sa[j][i] = (i < b1) ? l[t] : zero<T>();
+        Sb[i][j] = v[t];
+    }
+    __syncthreads();
+    const int ti = 4 * (tid & 15), tj = 4 * (tid >> 4);
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = zero<T>();
+    for (int k2 = 0; k2 < NB; ++k2) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = Sa[k2][ti + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = Sb[k2][tj + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+    // X10 = -X1 T1: Sa[k][i] = X1(i, k), Sb[k][j] = T1(k, j)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) Sb[ti + a][tj + b] = acc[a][b];
+#pragma unroll
+    for (int t = 0; t < PT; ++t) {
+        const int e = tid + 256 * t, i = e & (NB - 1), j = e / NB;
+        Sa[j][i] = w[t];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = zero<T>();
+    for (int k2 = 0; k2 < NB; ++k2) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = Sa[k2][ti + a];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = Sb[k2][tj + b];
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (ti + a < b1) d.LW[(NB + ti + a) + (int64_t)(tj + b) * n] = -acc[a][b];
+}
+
+template <class T>
+void launch_wide_inverse_small(const WideInvDesc<T>* d_descs, int ndesc, hipStream_t s) {
+    if (ndesc <= 0) return;
+    hipLaunchKernelGGL(k_wide_inverse_small<T>, dim3(ndesc), dim3(256), 0, s, d_descs);
+}
+
+// ---------------------------------------------------------------------------
 // Extend-add: one wave per pair of child update columns (j0 + p and
 // j0 + nc - 1 - p, so every wave adds about the same number of entries),
 // four waves per workgroup. Children of one slot have distinct parents, so no
@@ -1938,6 +2054,7 @@ void launch_fill_int(int* p, int n, int value, hipStream_t s) {
     template void launch_gemm<T>(const GemmDesc<T>*, const int*, int, int, int, int, hipStream_t);        \
     template void launch_extend_add<T>(const ExtendDesc<T>*, const int*, int, int, hipStream_t); \
     template void launch_diag_ldl<T>(const DiagDesc<T>*, int, bool, int*, hipStream_t);         \
+    template void launch_wide_inverse_small<T>(const WideInvDesc<T>*, int, hipStream_t);        \
     template void launch_gather<T>(const SolveSeg*, const int*, int, int, T*, const T*, int,    \
                                    int, const int*, hipStream_t);                               \
     template void launch_solve_extend<T>(const SolveExtend*, const int*, int, int, T*, T*, int, \

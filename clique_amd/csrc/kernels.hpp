@@ -242,6 +242,18 @@ struct PermSymDesc {
     int64_t ldy, lda;
     int n;
 };
+// The solves' inverted diagonal block of a front with n <= 2 NB columns in
+// one workgroup (launch_wide_inverse_small): LW = [X0 0; -X1 L10 X0 X1] with
+// X0, X1 the inverses of the unit-lower 64-blocks.
+template <class T>
+struct WideInvDesc {
+    const T* X0;     // inverse of L11's first 64-block (NB x NB, ld NB)
+    const T* X1;     // of its second block (rows and columns 64..n-1), if n > NB
+    const T* L10;    // frontL rows 64..n-1 of columns 0..63 (ld ldl)
+    T* LW;           // n x n, ld n
+    int64_t ldl;
+    int n;
+};
 // A front's rows [off, off + n) of the nodal right-hand sides
 struct PermSeg {
     int off, n;
@@ -252,6 +264,8 @@ void launch_bk_panel(const BkDesc<T>* d_descs, int ndesc, int nbp, bool herm, in
                      hipStream_t s);
 template <class T>
 void launch_unit_lower_inverse(const TriInvDesc<T>* d_descs, int ndesc, hipStream_t s);
+template <class T>
+void launch_wide_inverse_small(const WideInvDesc<T>* d_descs, int ndesc, hipStream_t s);
 // start[i] = workgroups before descriptor i (one per column and 256-row chunk)
 template <class T>
 void launch_gather_cols(const GatherColsDesc<T>* d_descs, const int* d_start, int ndesc,

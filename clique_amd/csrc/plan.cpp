@@ -146,7 +146,7 @@ struct IntervalAlloc {
 
 enum StepKind { S_EXTEND, S_DIAG, S_GEMM, S_GATHER, S_UNUSED, S_SEXTEND, S_ZERO, S_SOLVE_GEMM, S_COPY,
                 S_RECORD, S_WAIT, S_REDUCE, S_ZLOWER, S_BK, S_TRINV, S_GCOLS, S_QSCALE, S_PERM,
-                S_QROWS, S_PSYM, S_PSTEP };
+                S_QROWS, S_PSYM, S_PSTEP, S_WINV };
 
 struct Step {
     int kind;
@@ -315,6 +315,7 @@ const char* step_name(int kind) {
         case 18: return "quasi_rows";
         case 19: return "permute_sym";
         case 20: return "solve_panel";
+        case 21: return "wide_inverse";
         default: return "other";
     }
 }
@@ -1020,9 +1021,17 @@ struct Numeric : NumericBase {
         struct WB { int s, w0, sbw; int64_t tmp; };
         std::vector<WB> wbs;
         std::vector<ZeroSeg<T>> zs;
+        // fronts of at most min(SB, 2 NB) columns: their one wide block in
+        // one workgroup each (launch_wide_inverse_small)
+        std::vector<WideInvDesc<T>> small;
         int64_t tmpTotal = 0;
         for (int s : lv) {
             const FrontLayout& f = P.fronts[s];
+            if (f.n > 0 && f.n <= std::min(SB, 2 * NB)) {
+                small.push_back(WideInvDesc<T>{Li(s, 0), f.n > NB ? Li(s, NB) : Li(s, 0),
+                                               L(s) + NB, LW(s, 0), f.m, f.n});
+                continue;
+            }
             int64_t entries = 0;
             for (int w0 = 0; w0 < f.n; w0 += SB) {
                 const int sbw = std::min(SB, f.n - w0);
@@ -1032,6 +1041,8 @@ struct Numeric : NumericBase {
             }
             if (entries) zs.push_back(ZeroSeg<T>{LW(s, 0), entries});
         }
+        if (!small.empty())
+            sb.steps.push_back(Step{S_WINV, sb.arena.add(small), 0, (int)small.size(), 0, false});
         if (tmpTotal * (int64_t)sizeof(T) > (int64_t)wtmp.bytes)
             throw Error(5, "wide inverse workspace smaller than a level needs");
         sb.zero(zs);
@@ -1678,6 +1689,9 @@ struct Numeric : NumericBase {
             case S_PSYM:
                 launch_permute_sym<T>(a.at<PermSymDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
                                       s.nwork, st);
+                break;
+            case S_WINV:
+                launch_wide_inverse_small<T>(a.at<WideInvDesc<T>>(s.desc), s.ndesc, st);
                 break;
             case S_PSTEP:
                 launch_panel_step<T>(a.at<PanelStepDesc<T>>(s.desc), a.at<int>(s.start), s.ndesc,
