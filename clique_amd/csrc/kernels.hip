@@ -809,12 +809,21 @@ __global__ __launch_bounds__(256) void k_copy2d(const CopyDesc<T>* __restrict__ 
     const CopyDesc<T> d = descs[di];
     const int64_t total = (int64_t)d.rows * d.cols;
     const int64_t base = (int64_t)(g - chunk_start[di]) * 1024;
-    for (int i = threadIdx.x; i < 1024; i += blockDim.x) {
-        const int64_t e = base + i;
-        if (e >= total) break;
-        const int64_t r = e % d.rows, c = e / d.rows;
-        d.dst[r + c * d.ldd] = d.src[r + c * d.lds];
+    // all four loads before the stores (source and destination may share a
+    // pool, so the compiler would otherwise wait a round trip per entry)
+    T v[4];
+    int64_t at[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int64_t e = base + threadIdx.x + 256 * q;
+        const bool ok = e < total;
+        const int64_t r = ok ? e % d.rows : 0, c = ok ? e / d.rows : 0;
+        at[q] = ok ? r + c * d.ldd : -1;
+        v[q] = d.src[r + c * d.lds];
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (at[q] >= 0) d.dst[at[q]] = v[q];
 }
 
 template <class T>
