@@ -552,7 +552,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
     const int g = lane >> 4, li = lane & 15;
     const int ncb = (d.N + 15) >> 4;   // 16-column blocks of C
     const bool transA = d.flags & G_TRANSA, conjA = d.flags & G_CONJA;
-    const int K = d.K;
+    // K range of this row tile: a triangular op(A) (the solves' inverted
+    // blocks, whose other triangle is stored as zeros) bounds it by the
+    // tile's rows; K is the range's end, k0 runs from kbeg
+    int kbeg = 0, K = d.K;
+    if (d.flags & G_KLO) kbeg = min((row0 / KC) * KC, K);
+    if (d.flags & G_KHI) K = min(K, row0 + 64 * RB);
 
     // this lane's rows of op(A): op(A)(r, k) = A[r + k lda], or A[k + r lda]
     const int wrow0 = row0 + wave * 16 * RB;
@@ -631,12 +636,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
         }
     };
 
-    loadA(0, ra);
-    loadB(0);
-    storeB(0);
+    if (kbeg < K) {
+        loadA(kbeg, ra);
+        loadB(kbeg);
+        storeB(0);
+    }
     __syncthreads();
     int buf = 0;
-    for (int k0 = 0; k0 < K; k0 += KC) {
+    for (int k0 = kbeg; k0 < K; k0 += KC) {
         const bool more = k0 + KC < K;
         if (more) {
             loadA(k0 + KC, rn);

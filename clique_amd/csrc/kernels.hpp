@@ -58,8 +58,9 @@ enum GemmFlags : int {
 };
 // 16-row blocks per wave of a streaming tile (four waves: 64 x this rows)
 constexpr int STREAM_ROW_BLOCKS = 1;   // two measured slower on the 120^3 PML solves (28.9 ms vs 24.7)
-// Flags a streaming tile accepts (a plain or transposed A, no masks)
-constexpr int STREAM_FLAGS = G_TRANSA | G_CONJA | G_ACCUM | G_NEG;
+// Flags a streaming tile accepts (a plain or transposed A, a triangular K
+// range, no output masks)
+constexpr int STREAM_FLAGS = G_TRANSA | G_CONJA | G_ACCUM | G_NEG | G_KLO | G_KHI;
 
 // C(MxN) (+)= +-op(A)(MxK) diag(scaleK) op(B)(KxN) [diag(scaleN)^{-1}], where
 // op(A) is A, A^T, conj(A) or A^H per the flags (scaleK is never conjugated).

@@ -447,7 +447,8 @@ struct StepBuilder {
                 for (const auto& d : ds)
                     if (d.flags & (G_KLO | G_KHI)) ok = false;
             // streaming tiles: solve products of at most 64 right-hand
-            // sides, plain or transposed A, no masks, scales or aliasing
+            // sides, plain or transposed A (a triangular K range only on the
+            // solves' inverted blocks), no output masks, scales or aliasing
             if (v == TILE_STREAM) {
                 ok = ok && solve && !scaled;
                 for (const auto& d : ds)
