@@ -212,6 +212,7 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 2048) ? 2
     constexpr int LA = TM * TK / NT, LB = TN * TK / NT;
     constexpr bool F64 = sizeof(R) == 8;
     constexpr bool SCALED = ROLE == 2;   // every descriptor of the launch has scaleK
+    constexpr bool PRIO = !(ROLE == 3 && !F64);   // raised priority over the MFMA loop
 
     __shared__ R As[2][NP][TK][TM + LPAD];
     __shared__ R Bs[2][NP][TK][TN + LPAD];
@@ -378,10 +379,11 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 2048) ? 2
     auto chunk = [&](int c, auto bufc) {
         constexpr int buf = decltype(bufc)::value;
         if (c + 1 < nk) load(kbeg + (c + 1) * TK);
-        // raised wave priority while feeding the MFMAs (measured: +3 % for
-        // complex FP64, neutral elsewhere); the TRSMs, on the factorization's
+        // raised wave priority while feeding the MFMAs (measured on the
+        // 120^3 complex factorizations: FP64 updates 1.5 % faster with it,
+        // FP32 updates 1 % faster without); the TRSMs, on the factorization's
         // critical path beside the lookahead updates, one level higher
-        __builtin_amdgcn_s_setprio(ROLE == 0 ? 2 : 1);
+        if constexpr (PRIO) __builtin_amdgcn_s_setprio(ROLE == 0 ? 2 : 1);
 #pragma unroll
         for (int kk = 0; kk < TK / 4; ++kk) {
             const int kr = kk * 4 + (lane >> 4);
@@ -440,7 +442,7 @@ __attribute__((amdgpu_waves_per_eu((TM * TN / (WM * WN) >= 2048) ? 2
                     }
             }
         }
-        __builtin_amdgcn_s_setprio(ROLE == 0 ? 1 : 0);
+        if constexpr (PRIO) __builtin_amdgcn_s_setprio(ROLE == 0 ? 1 : 0);
         if (c + 1 < nk) store(std::integral_constant<int, buf ^ 1>{});
         __syncthreads();
     };
